@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Forward + adjoint throughput of the cone-beam operators on one MI355X.
+
+Workload (default): a 256^3 fp32 phantom of 12 random ellipsoids, A=720 views,
+768x640 detector, pixel 0.6 mm, SOD 600 mm, SID 1100 mm, voxel 0.5 mm, step
+0.25 mm, per-view geometry perturbed (angle N(0,0.2 deg), centres N(0,1.5 px),
+source N(0,0.5 mm), distances x(1+N(0,0.01))).  One step = one forward
+projection plus one adjoint (volume gradient and the eight per-view geometry
+gradients) for a fixed upstream gradient, inputs resident in HBM.
+
+Prints ONE JSON line:
+  value      GUPS = 2*D*H*W*A / (t_fwd + t_adj) / 1e9
+  roofline   algorithmic tap traffic of the dominant kernel (forward: 32 B per
+             trilinear sample + 4 B per ray; adjoint: 64 B per sample + 4 B per
+             ray + 4 B per voxel) over its HIP-event duration, against 8 TB/s
+  cpu_baseline  the FP32 CPU oracle (OpenMP) on a bounded subset of views.
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "diff-geocali_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "fwd+adjoint GUPS at 256³×720 views; calib s to 0.1 px, learned vs plain"
+HBM_PEAK_GBPS = 8000.0
+
+WORKLOADS = {
+    "geometry-gradients-256c-720v": dict(D=256, A=720, U=768, V=640, pixel=0.6, sod=600.0, sid=1100.0,
+                                         voxel=0.5, step=0.25, perturb=True),
+    "fdk-roundtrip-256c-360v": dict(D=256, A=360, U=512, V=384, pixel=0.8, sod=500.0, sid=1000.0,
+                                    voxel=0.5, step=0.25, perturb=False),
+}
+DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
+PHANTOM_SEED = 20261203
+PERTURB_SEED = 7
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
+    return p.parse_args()
+
+
+def make_inputs(w, device):
+    import numpy as np
+    import torch
+
+    from cbct_backprojector import phantom
+    D = w["D"]
+    half = D * w["voxel"] / 2
+    params = phantom.ellipsoid_parameters(PHANTOM_SEED, 12, half_width=half)
+    vol_np = phantom.rasterize_ellipsoids(params, (D, D, D), w["voxel"])
+    geom = phantom.circular_geometry(w["A"], w["sid"], w["sod"], w["U"], w["V"], w["pixel"])
+    if w["perturb"]:
+        geom = phantom.perturb_geometry(geom, PERTURB_SEED)
+    glist = [geom[k].contiguous() for k in phantom.GEOMETRY_NAMES]
+    vol = torch.tensor(vol_np, device=device)
+    gdev = [g.to(device) for g in glist]
+    gen = torch.Generator(device=device).manual_seed(11)
+    upstream = torch.randn((w["A"], w["U"], w["V"]), generator=gen, device=device, dtype=torch.float32)
+    return vol_np, [g.numpy() for g in glist], vol, gdev, upstream
+
+
+def copy_rate_gbps(torch, device):
+    n = 1 << 28   # 1 GiB of float32
+    a = torch.empty(n, dtype=torch.float32, device=device).fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / 5 / 1e3
+    del a, b
+    return 2 * n * 4 / t / 1e9
+
+
+def cpu_baseline(w, vol_np, geom_np, budget_s):
+    """FP32 CPU oracle forward + adjoint on uniformly spaced views until the budget is used."""
+    import numpy as np
+
+    import oracle
+    A = w["A"]
+    order = [int(round(i * A / 8)) % A for i in range(8)]
+    threads = oracle.max_threads()
+    rng = np.random.default_rng(11)
+    done, t_total = 0, 0.0
+    for a in order:
+        g1 = [np.ascontiguousarray(g[a:a + 1]) for g in geom_np]
+        up = rng.standard_normal((1, w["U"], w["V"])).astype(np.float32)
+        t0 = time.perf_counter()
+        oracle.fp_forward(vol_np, g1, w["pixel"], w["U"], w["V"], w["step"], w["voxel"], mode="f32",
+                          threads=threads)
+        oracle.fp_backward(up, vol_np, g1, w["pixel"], w["step"], w["voxel"], mode="f32", threads=threads)
+        t_total += time.perf_counter() - t0
+        done += 1
+        if t_total >= budget_s:
+            break
+    D = w["D"]
+    gups = 2.0 * D * D * D * done / t_total / 1e9
+    return {"value": gups, "unit": "GUPS", "cores": threads, "kind": "port",
+            "sample": f"{done} of {A} views (uniformly spaced), forward + adjoint with volume gradient, "
+                      f"FP32 reference-schedule C oracle, {t_total:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the operators run on a single device; multi-GPU sharding is design-only).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from cbct_backprojector import _native
+    from cbct_backprojector.source_only_forward_projector import project  # noqa: F401
+
+    device = torch.device("cuda:0")
+    torch.cuda.set_device(device)
+    w = WORKLOADS[args.workload]
+    vol_np, geom_np, vol, geom, upstream = make_inputs(w, device)
+    D, A, U, V = w["D"], w["A"], w["U"], w["V"]
+    scal = (w["pixel"], U, V, w["step"], w["voxel"], 0.0, 0.0, 0.0)
+    S = _native.fp_count_samples((D, D, D), geom, w["pixel"], U, V, w["step"], w["voxel"])
+
+    def step(events=None):
+        if events is not None:
+            events[0].record()
+        proj = _native.fp_forward(vol, geom, *scal)
+        if events is not None:
+            events[1].record()
+        gvol, ggeo = _native.fp_backward(upstream, vol, geom, w["pixel"], w["step"], w["voxel"], True,
+                                         0.0, 0.0, 0.0)
+        if events is not None:
+            events[2].record()
+        return proj, gvol, ggeo
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_fwd = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps / 1e3
+    t_adj = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps / 1e3
+    N = D * D * D * A
+    gups = 2.0 * N / (t_fwd + t_adj) / 1e9
+    rays = A * U * V
+    b_fwd = 32.0 * S + 4.0 * rays
+    b_adj = 64.0 * S + 4.0 * rays + 4.0 * D * D * D
+    dominant = ("adjoint", b_adj, t_adj) if t_adj >= t_fwd else ("forward", b_fwd, t_fwd)
+    achieved = dominant[1] / dominant[2] / 1e9
+    try:
+        copy_bw = copy_rate_gbps(torch, device)
+    except Exception:  # pragma: no cover
+        copy_bw = None
+    out = {
+        "metric": METRIC,
+        "value": gups,
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "volume": [D, D, D], "views": A, "detector": [U, V],
+                   "pixel_mm": w["pixel"], "sod_mm": w["sod"], "sid_mm": w["sid"], "voxel_mm": w["voxel"],
+                   "step_mm": w["step"], "geometry_perturbed": w["perturb"], "samples": S,
+                   "t_fwd_ms": t_fwd * 1e3, "t_adj_ms": t_adj * 1e3},
+        "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "pair_frac": (b_fwd + b_adj) / (t_fwd + t_adj) / 1e9 / HBM_PEAK_GBPS,
+                     "copy_GBps": copy_bw},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, vol_np, geom_np, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,213 @@
+"""ctypes binding of libcbct_hip.so, the gfx950 operator library.
+
+The library has a plain C ABI (``include/cbct_hip.h``): device pointers, int64
+sizes, float scalars, a caller-owned workspace and a stream.  This module
+allocates outputs and workspaces with torch on the input's device, passes
+torch's current stream, and turns a non-zero status into ``RuntimeError`` with
+the library's message.  There is no other implementation behind it: when the
+library is missing every operator raises.
+"""
+
+import ctypes
+import os
+
+import torch
+
+_LIB = None
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "libcbct_hip.so")
+
+_BUILD_HELP = (
+    "The HIP operator library is not built ({path} is missing). From the repository "
+    "root run: make -C diff-geocali_amd/csrc  (or python -c \"import __graft_entry__ as g; g.build()\")"
+)
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_F = ctypes.c_float
+_SZ = ctypes.c_size_t
+_INT = ctypes.c_int
+
+_SIGNATURES = {
+    "cbct_version": ([], ctypes.c_char_p),
+    "cbct_last_error": ([], ctypes.c_char_p),
+    "cbct_fp_workspace_bytes": ([_I64, _I64, _I64], _SZ),
+    "cbct_fp_forward": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_I64, _F, _I64, _I64, _F, _F, _F, _F, _F,
+                        _P, _P, _SZ, _P], _INT),
+    "cbct_fp_backward": ([_P, _P, _I64, _I64, _I64] + [_P] * 8 + [_I64, _I64, _I64, _F, _F, _F, _INT,
+                         _F, _F, _F, _P, _P, _P, _SZ, _P], _INT),
+    "cbct_fp_count_samples": ([_I64, _I64, _I64] + [_P] * 8 + [_I64, _F, _I64, _I64, _F, _F, _F, _F, _F,
+                              _P, _P, _SZ, _P], _INT),
+    "cbct_bp_workspace_bytes": ([_I64, _I64, _I64, _I64], _SZ),
+    "cbct_bp_forward": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
+                        _P, _P, _SZ, _P], _INT),
+    "cbct_bp_backward": ([_P, _P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
+                         _P, _P, _P, _SZ, _P], _INT),
+    "cbct_fdk_workspace_bytes": ([_I64, _I64, _I64], _SZ),
+    "cbct_fdk_filter": ([_P, _I64, _I64, _I64, _F, _P, _P, _F, _P, _P, _SZ, _P], _INT),
+    "cbct_fdk_reconstruct": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
+                             _F, _I64, _P, _P, _SZ, _P], _INT),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+
+def lib():
+    """Load (once) and return the operator library; raise RuntimeError when it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBRARY_PATH):
+            raise RuntimeError(_BUILD_HELP.format(path=LIBRARY_PATH))
+        handle = ctypes.CDLL(LIBRARY_PATH)
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = handle
+    return _LIB
+
+
+def version():
+    return lib().cbct_version().decode()
+
+
+def _check(status):
+    if status != 0:
+        raise RuntimeError(lib().cbct_last_error().decode())
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _workspace(nbytes, device):
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def _require_gpu(t, what):
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise RuntimeError(f"{what} must be a GPU tensor (ROCm devices appear as 'cuda')")
+
+
+def fp_forward(volume, geometry, pixel_size, U, V, step_size, voxel_size, off_x, off_y, off_z):
+    """proj[A,U,V]; volume [D,H,W] and the 8 geometry tensors must be contiguous float32."""
+    _require_gpu(volume, "volume")
+    L = lib()
+    D, H, W = volume.shape
+    A = geometry[0].numel()
+    dev = volume.device
+    with torch.cuda.device(dev):
+        out = torch.empty((A, U, V), dtype=torch.float32, device=dev)
+        ws_bytes = L.cbct_fp_workspace_bytes(A, U, V)
+        ws = _workspace(ws_bytes, dev)
+        _check(L.cbct_fp_forward(_ptr(volume), D, H, W, *[_ptr(g) for g in geometry], A,
+                                 pixel_size, U, V, step_size, voxel_size, off_x, off_y, off_z,
+                                 _ptr(out), _ptr(ws), ws_bytes, _stream(dev)))
+    return out
+
+
+def fp_backward(grad, volume, geometry, pixel_size, step_size, voxel_size, compute_grad_volume,
+                off_x, off_y, off_z):
+    """(gvol [D,H,W] or empty, 8 geometry gradients [A]) of sum(grad * fp_forward)."""
+    _require_gpu(volume, "volume")
+    L = lib()
+    D, H, W = volume.shape
+    A, U, V = grad.shape
+    dev = volume.device
+    with torch.cuda.device(dev):
+        gvol = (torch.empty((D, H, W), dtype=torch.float32, device=dev) if compute_grad_volume
+                else torch.empty((0,), dtype=torch.float32, device=dev))
+        ggeo = torch.empty((8, A), dtype=torch.float32, device=dev)
+        ws_bytes = L.cbct_fp_workspace_bytes(A, U, V)
+        ws = _workspace(ws_bytes, dev)
+        _check(L.cbct_fp_backward(_ptr(grad), _ptr(volume), D, H, W, *[_ptr(g) for g in geometry],
+                                  A, U, V, pixel_size, step_size, voxel_size,
+                                  1 if compute_grad_volume else 0, off_x, off_y, off_z,
+                                  _ptr(gvol) if compute_grad_volume else None, _ptr(ggeo),
+                                  _ptr(ws), ws_bytes, _stream(dev)))
+    return gvol, ggeo
+
+
+def fp_count_samples(volume_shape, geometry, pixel_size, U, V, step_size, voxel_size,
+                     off_x=0.0, off_y=0.0, off_z=0.0):
+    """Exact number of trilinear samples the forward schedule takes (int)."""
+    dev = geometry[0].device
+    _require_gpu(geometry[0], "geometry")
+    L = lib()
+    D, H, W = volume_shape
+    A = geometry[0].numel()
+    with torch.cuda.device(dev):
+        count = torch.zeros(1, dtype=torch.int64, device=dev)
+        ws_bytes = L.cbct_fp_workspace_bytes(A, U, V)
+        ws = _workspace(ws_bytes, dev)
+        _check(L.cbct_fp_count_samples(D, H, W, *[_ptr(g) for g in geometry], A, pixel_size, U, V,
+                                       step_size, voxel_size, off_x, off_y, off_z, _ptr(count),
+                                       _ptr(ws), ws_bytes, _stream(dev)))
+    return int(count.item())
+
+
+def bp_forward(proj, geometry, pixel_size, D, H, W, voxel_size, off_x, off_y, off_z):
+    _require_gpu(proj, "projections")
+    L = lib()
+    A, U, V = proj.shape
+    dev = proj.device
+    with torch.cuda.device(dev):
+        vol = torch.empty((D, H, W), dtype=torch.float32, device=dev)
+        ws_bytes = L.cbct_bp_workspace_bytes(A, D, H, W)
+        ws = _workspace(ws_bytes, dev)
+        _check(L.cbct_bp_forward(_ptr(proj), A, U, V, *[_ptr(g) for g in geometry], pixel_size,
+                                 D, H, W, voxel_size, off_x, off_y, off_z, _ptr(vol), _ptr(ws),
+                                 ws_bytes, _stream(dev)))
+    return vol
+
+
+def bp_backward(grad_vol, proj, geometry, pixel_size, voxel_size, off_x, off_y, off_z):
+    _require_gpu(proj, "projections")
+    L = lib()
+    A, U, V = proj.shape
+    D, H, W = grad_vol.shape
+    dev = proj.device
+    with torch.cuda.device(dev):
+        gproj = torch.empty((A, U, V), dtype=torch.float32, device=dev)
+        ggeo = torch.empty((8, A), dtype=torch.float32, device=dev)
+        ws_bytes = L.cbct_bp_workspace_bytes(A, D, H, W)
+        ws = _workspace(ws_bytes, dev)
+        _check(L.cbct_bp_backward(_ptr(grad_vol), _ptr(proj), A, U, V, *[_ptr(g) for g in geometry],
+                                  pixel_size, D, H, W, voxel_size, off_x, off_y, off_z,
+                                  _ptr(gproj), _ptr(ggeo), _ptr(ws), ws_bytes, _stream(dev)))
+    return gproj, ggeo
+
+
+def fdk_filter(proj, pixel_size, sod, sid, alpha):
+    """sod, sid: contiguous float32 [A] tensors on the projections' device."""
+    _require_gpu(proj, "projections")
+    L = lib()
+    A, U, V = proj.shape
+    dev = proj.device
+    with torch.cuda.device(dev):
+        out = torch.empty((A, U, V), dtype=torch.float32, device=dev)
+        ws_bytes = L.cbct_fdk_workspace_bytes(A, U, V)
+        ws = _workspace(ws_bytes, dev)
+        _check(L.cbct_fdk_filter(_ptr(proj), A, U, V, pixel_size, _ptr(sod), _ptr(sid), alpha,
+                                 _ptr(out), _ptr(ws), ws_bytes, _stream(dev)))
+    return out
+
+
+def fdk_reconstruct(proj, geometry, pixel_size, D, H, W, voxel_size, off_x, off_y, off_z, alpha,
+                    angle_chunk_size):
+    _require_gpu(proj, "projections")
+    L = lib()
+    A, U, V = proj.shape
+    dev = proj.device
+    with torch.cuda.device(dev):
+        vol = torch.empty((D, H, W), dtype=torch.float32, device=dev)
+        ws_bytes = L.cbct_fdk_workspace_bytes(A, U, V)
+        ws = _workspace(ws_bytes, dev)
+        _check(L.cbct_fdk_reconstruct(_ptr(proj), A, U, V, *[_ptr(g) for g in geometry], pixel_size,
+                                      D, H, W, voxel_size, off_x, off_y, off_z, alpha,
+                                      angle_chunk_size, _ptr(vol), _ptr(ws), ws_bytes, _stream(dev)))
+    return vol

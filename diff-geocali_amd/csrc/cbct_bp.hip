@@ -1,0 +1,319 @@
+// Weighted voxel-driven backprojector and its backward for gfx950.
+//
+// Work assignment: one wave owns one (x, y) voxel column and 64*K voxels of it
+// along z (lane = z, K voxels per lane).  Along a column only the detector row
+// v changes, and v is the contiguous axis of the [A,U,V] projections, so the
+// four bilinear taps of a wave fall in a few cache lines.  All view-dependent
+// terms that do not depend on z (rotation, magnification t, u index, weight)
+// are evaluated once per lane and view and reused for its K voxels.  The
+// per-view constants are wave-uniform and read through the scalar cache.
+//
+// Arithmetic follows csrc/src_cbct_backprojector_cuda.cu:37-95 (forward) and
+// :132-292 (backward) in the FP32 operation order fixed here and in the oracle.
+#include "cbct_common.h"
+
+#include <algorithm>
+
+namespace cbct {
+
+constexpr int BP_WAVES = 4;  // waves (columns) per workgroup
+
+struct BpGeom {
+    int U, V, D, H, W, A;
+    float pix, inv_pix, vs, offx, offy, offz, hx, hy, hz;
+};
+
+// out = scale * sum over chunks of (sum over the chunk's views), clamped at 0
+// when clamp is set.  chunk = A gives the plain backprojection.
+template <int K>
+__global__ __launch_bounds__(64 * BP_WAVES) void bp_forward_kernel(
+    const float* __restrict__ proj, const ViewFrame* __restrict__ frames, BpGeom g,
+    int chunk, float scale, int clamp, float* __restrict__ vol) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = __builtin_amdgcn_readfirstlane(blockIdx.x * BP_WAVES + wave);
+    if (col >= g.H * g.W) return;
+    const int x = col % g.W, y = col / g.W;
+    const int zb = blockIdx.y * 64 * K + lane;
+    const float vx = fmaf((float)x - g.hx, g.vs, g.offx);
+    const float vy = fmaf((float)y - g.hy, g.vs, g.offy);
+    float vz[K], total[K], acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        vz[k] = fmaf((float)(zb + 64 * k) - g.hz, g.vs, g.offz);
+        total[k] = 0.f;
+        acc[k] = 0.f;
+    }
+    const float Um1 = (float)g.U - 1.0f, Vm1 = (float)g.V - 1.0f;
+    int next = chunk;
+    for (int a = 0; a < g.A; ++a) {
+        if (a == next) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) { total[k] += acc[k]; acc[k] = 0.f; }
+            next += chunk;
+        }
+        const ViewFrame f = frames[a];
+        const float px = fmaf(vx, f.c, vy * f.s);
+        const float py = fmaf(-vx, f.s, vy * f.c);
+        const float m = f.sod + f.dsn;
+        const float q = m - py;
+        const float l = f.sid + f.dsn;
+        if (q <= 1e-6f || l <= 1e-6f) continue;
+        const float t = l / q;
+        const float uph = fmaf(t, px - f.dsu, f.dsu);
+        const float sw = m / q;
+        const float gw = sw * sw;
+        const float ui = uph / g.pix + f.u0;
+        if (!(ui >= 0.0f && ui < Um1)) continue;
+        const int iu = (int)floorf(ui);
+        const float fu = ui - (float)iu, mu = 1.0f - fu;
+        const float* row0 = proj + ((int64_t)a * g.U + iu) * g.V;
+        const float* row1 = row0 + g.V;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const float vph = fmaf(t, vz[k] - f.dsv, f.dsv);
+            const float vi = vph / g.pix + f.v0;
+            if (vi >= 0.0f && vi < Vm1) {
+                const int iv = (int)floorf(vi);
+                const float fv = vi - (float)iv, mv = 1.0f - fv;
+                const float p00 = row0[iv], p01 = row0[iv + 1], p10 = row1[iv], p11 = row1[iv + 1];
+                const float interp = fmaf(fu * fv, p11, fmaf(mu * fv, p01, fmaf(fu * mv, p10, mu * mv * p00)));
+                acc[k] = fmaf(gw, interp, acc[k]);
+            }
+        }
+    }
+    const int64_t plane = (int64_t)g.H * g.W;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int z = zb + 64 * k;
+        if (z < g.D) {
+            float r = (total[k] + acc[k]) * scale;
+            if (clamp) r = fmaxf(r, 0.0f);
+            vol[z * plane + (int64_t)y * g.W + x] = r;
+        }
+    }
+}
+
+// Backward: per voxel-view, the bilinear transpose scatters into gproj and the
+// analytic derivatives of weight*interp feed the eight geometry sums, which
+// are reduced over the wave and added once per wave and view.
+template <int K>
+__global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
+    const float* __restrict__ gvol, const float* __restrict__ proj,
+    const ViewFrame* __restrict__ frames, BpGeom g,
+    float* __restrict__ gproj, float* __restrict__ ggeom) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = __builtin_amdgcn_readfirstlane(blockIdx.x * BP_WAVES + wave);
+    if (col >= g.H * g.W) return;
+    const int x = col % g.W, y = col / g.W;
+    const int zb = blockIdx.y * 64 * K + lane;
+    const float vx = fmaf((float)x - g.hx, g.vs, g.offx);
+    const float vy = fmaf((float)y - g.hy, g.vs, g.offy);
+    const int64_t plane = (int64_t)g.H * g.W;
+    float vz[K], gv[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int z = zb + 64 * k;
+        vz[k] = fmaf((float)z - g.hz, g.vs, g.offz);
+        gv[k] = z < g.D ? gvol[z * plane + (int64_t)y * g.W + x] : 0.0f;
+    }
+    const float Um1 = (float)g.U - 1.0f, Vm1 = (float)g.V - 1.0f;
+    const float ip = g.inv_pix;
+    for (int a = 0; a < g.A; ++a) {
+        const ViewFrame f = frames[a];
+        const float px = fmaf(vx, f.c, vy * f.s);
+        const float py = fmaf(-vx, f.s, vy * f.c);
+        const float m = f.sod + f.dsn;
+        const float q = m - py;
+        const float l = f.sid + f.dsn;
+        if (q <= 1e-6f || l <= 1e-6f) continue;
+        const float iq = 1.0f / q, iq2 = iq * iq, iq3 = iq2 * iq;
+        const float t = l * iq;
+        const float du = px - f.dsu;
+        const float uph = fmaf(t, du, f.dsu);
+        const float m2 = m * m;
+        const float gw = m2 * iq2;
+        const float ui = fmaf(uph, ip, f.u0);
+        if (!(ui >= 0.0f && ui < Um1)) continue;
+        const int iu = (int)floorf(ui);
+        const float fu = ui - (float)iu, mu = 1.0f - fu;
+        const int64_t base = ((int64_t)a * g.U + iu) * g.V;
+        // z-independent derivative factors
+        const float dt_sod = -l * iq2;
+        const float dW_m = 2.0f * m * iq2 - 2.0f * m2 * iq3;     // d(m^2/q^2)/dm at fixed q-m
+        const float dsu_b = f.dsn, dsn_b = -f.dsu;
+        const float dq_b = dsn_b + px;
+        const float dt_b = (dsn_b * q - l * dq_b) * iq2;
+        const float dU_b = fmaf(du, dt_b, fmaf(1.0f - t, dsu_b, t * py));
+        const float dW_b = 2.0f * m * iq2 * dsn_b - 2.0f * m2 * iq3 * dq_b;
+        const float qml = q - l;
+        const float dt_sx = -f.s * qml * iq2;
+        const float dU_sx = fmaf(du, dt_sx, (1.0f - t) * f.c);
+        const float dW_sx = dW_m * -f.s;
+        const float dt_sy = f.c * qml * iq2;
+        const float dU_sy = fmaf(du, dt_sy, (1.0f - t) * f.s);
+        const float dW_sy = dW_m * f.c;
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const float dv = vz[k] - f.dsv;
+            const float vph = fmaf(t, dv, f.dsv);
+            const float vi = fmaf(vph, ip, f.v0);
+            if (zb + 64 * k < g.D && vi >= 0.0f && vi < Vm1) {
+                const int iv = (int)floorf(vi);
+                const float fv = vi - (float)iv, mv = 1.0f - fv;
+                const int64_t k00 = base + iv;
+                const float p00 = proj[k00], p01 = proj[k00 + 1];
+                const float p10 = proj[k00 + g.V], p11 = proj[k00 + g.V + 1];
+                const float interp = fmaf(fu * fv, p11, fmaf(mu * fv, p01, fmaf(fu * mv, p10, mu * mv * p00)));
+                const float Iu = fmaf(fv, p11 - p01, mv * (p10 - p00));
+                const float Iv = fmaf(fu, p11 - p10, mu * (p01 - p00));
+                const float gb = gv[k] * gw;
+                atomicAdd(gproj + k00, gb * mu * mv);
+                atomicAdd(gproj + k00 + g.V, gb * fu * mv);
+                atomicAdd(gproj + k00 + 1, gb * mu * fv);
+                atomicAdd(gproj + k00 + g.V + 1, gb * fu * fv);
+                // sid (:214-219)
+                s1 += gb * fmaf(Iv, dv * iq * ip, Iu * du * iq * ip);
+                // sod (:222-230)
+                s2 += gv[k] * fmaf(gw, fmaf(Iv, dv * dt_sod * ip, Iu * du * dt_sod * ip), dW_m * interp);
+                // beta (:233-250)
+                s0 += gv[k] * fmaf(gw, fmaf(Iv, dv * dt_b * ip, Iu * dU_b * ip), dW_b * interp);
+                // detector centres (:252-253)
+                s3 += gb * Iu;
+                s4 += gb * Iv;
+                // source offsets (:255-292)
+                s5 += gv[k] * fmaf(gw, fmaf(Iv, dv * dt_sx * ip, Iu * dU_sx * ip), dW_sx * interp);
+                s6 += gv[k] * fmaf(gw, fmaf(Iv, dv * dt_sy * ip, Iu * dU_sy * ip), dW_sy * interp);
+                s7 += gb * Iv * (1.0f - t) * ip;
+            }
+        }
+        s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
+        s4 = wave_sum(s4); s5 = wave_sum(s5); s6 = wave_sum(s6); s7 = wave_sum(s7);
+        if (lane == 0) {
+            atomicAdd(ggeom + 0 * g.A + a, s0); atomicAdd(ggeom + 1 * g.A + a, s1);
+            atomicAdd(ggeom + 2 * g.A + a, s2); atomicAdd(ggeom + 3 * g.A + a, s3);
+            atomicAdd(ggeom + 4 * g.A + a, s4); atomicAdd(ggeom + 5 * g.A + a, s5);
+            atomicAdd(ggeom + 6 * g.A + a, s6); atomicAdd(ggeom + 7 * g.A + a, s7);
+        }
+    }
+}
+
+static int check_bp_common(int64_t A, int64_t U, int64_t V, int64_t D, int64_t H, int64_t W,
+                           float pix, float vs, float ox, float oy, float oz) {
+    CBCT_CHECK_ARG(A >= 1 && U >= 2 && V >= 2, "proj must have shape [A,U,V] with A >= 1 and U,V >= 2");
+    CBCT_CHECK_ARG(A * U * V <= INT32_MAX, "proj exceeds the kernel's 32-bit indexing limit");
+    CBCT_CHECK_ARG(D > 0 && H > 0 && W > 0 && D <= INT32_MAX && H <= INT32_MAX && W <= INT32_MAX,
+                   "D,H,W must be positive and fit in a 32-bit integer");
+    CBCT_CHECK_ARG(D <= INT32_MAX / H && D * H <= INT32_MAX / W, "volume exceeds the kernel's 32-bit indexing limit");
+    int st;
+    if ((st = check_positive_finite(pix, "pixel_size"))) return st;
+    if ((st = check_positive_finite(vs, "voxel_size"))) return st;
+    if ((st = check_finite(ox, "off_x"))) return st;
+    if ((st = check_finite(oy, "off_y"))) return st;
+    if ((st = check_finite(oz, "off_z"))) return st;
+    return CBCT_OK;
+}
+
+static BpGeom make_bp_geom(int64_t A, int64_t U, int64_t V, int64_t D, int64_t H, int64_t W,
+                           float pix, float vs, float ox, float oy, float oz) {
+    BpGeom g;
+    g.A = (int)A; g.U = (int)U; g.V = (int)V; g.D = (int)D; g.H = (int)H; g.W = (int)W;
+    g.pix = pix; g.inv_pix = 1.0f / pix; g.vs = vs;
+    g.offx = ox; g.offy = oy; g.offz = oz;
+    g.hx = (float)W * 0.5f; g.hy = (float)H * 0.5f; g.hz = (float)D * 0.5f;
+    return g;
+}
+
+static int bp_k(int64_t D) { return D > 128 ? 4 : (D > 64 ? 2 : 1); }
+
+int launch_bp_forward(const float* proj, const ViewFrame* frames, const BpGeom& g, int chunk,
+                      float scale, int clamp, float* vol, hipStream_t stream) {
+    const int K = bp_k(g.D);
+    dim3 grid((unsigned)ceil_div((int64_t)g.H * g.W, BP_WAVES), (unsigned)ceil_div(g.D, 64 * K));
+    if (K == 4)
+        bp_forward_kernel<4><<<grid, 64 * BP_WAVES, 0, stream>>>(proj, frames, g, chunk, scale, clamp, vol);
+    else if (K == 2)
+        bp_forward_kernel<2><<<grid, 64 * BP_WAVES, 0, stream>>>(proj, frames, g, chunk, scale, clamp, vol);
+    else
+        bp_forward_kernel<1><<<grid, 64 * BP_WAVES, 0, stream>>>(proj, frames, g, chunk, scale, clamp, vol);
+    CBCT_LAUNCHED("backprojection kernel launch");
+    return CBCT_OK;
+}
+
+int launch_bp_forward_fdk(const float* proj, const ViewFrame* frames, int64_t A, int64_t U, int64_t V,
+                          int64_t D, int64_t H, int64_t W, float pix, float vs, float ox, float oy,
+                          float oz, int chunk, float scale, float* vol, hipStream_t stream) {
+    const BpGeom g = make_bp_geom(A, U, V, D, H, W, pix, vs, ox, oy, oz);
+    return launch_bp_forward(proj, frames, g, chunk, scale, 1, vol, stream);
+}
+
+}  // namespace cbct
+
+using namespace cbct;
+
+extern "C" size_t cbct_bp_workspace_bytes(int64_t A, int64_t D, int64_t H, int64_t W) {
+    (void)D; (void)H; (void)W;
+    if (A < 1) return 0;
+    return (size_t)A * sizeof(ViewFrame);
+}
+
+extern "C" int cbct_bp_forward(const float* projections, int64_t A, int64_t U, int64_t V,
+                               const float* angles, const float* sid, const float* sod,
+                               const float* det_u0, const float* det_v0,
+                               const float* src_x, const float* src_y, const float* src_z,
+                               float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
+                               float off_x, float off_y, float off_z,
+                               float* volume, void* workspace, size_t workspace_bytes, void* stream_) {
+    int st;
+    if ((st = check_bp_common(A, U, V, D, H, W, pixel_size, voxel_size, off_x, off_y, off_z))) return st;
+    const float* gg[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    if ((st = check_geometry(gg, A))) return st;
+    CBCT_CHECK_ARG(projections != nullptr && volume != nullptr, "projections and volume must be non-null");
+    CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= cbct_bp_workspace_bytes(A, D, H, W),
+                   "workspace too small for cbct_bp_forward");
+    hipStream_t stream = (hipStream_t)stream_;
+    ViewFrame* frames = (ViewFrame*)workspace;
+    const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    if ((st = launch_frames(gp, A, frames, stream))) return st;
+    const BpGeom g = make_bp_geom(A, U, V, D, H, W, pixel_size, voxel_size, off_x, off_y, off_z);
+    return launch_bp_forward(projections, frames, g, (int)A, 1.0f, 0, volume, stream);
+}
+
+extern "C" int cbct_bp_backward(const float* grad_volume, const float* projections,
+                                int64_t A, int64_t U, int64_t V,
+                                const float* angles, const float* sid, const float* sod,
+                                const float* det_u0, const float* det_v0,
+                                const float* src_x, const float* src_y, const float* src_z,
+                                float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
+                                float off_x, float off_y, float off_z,
+                                float* grad_projections, float* grad_geometry,
+                                void* workspace, size_t workspace_bytes, void* stream_) {
+    int st;
+    if ((st = check_bp_common(A, U, V, D, H, W, pixel_size, voxel_size, off_x, off_y, off_z))) return st;
+    const float* gg[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    if ((st = check_geometry(gg, A))) return st;
+    CBCT_CHECK_ARG(grad_volume != nullptr && projections != nullptr && grad_projections != nullptr &&
+                   grad_geometry != nullptr, "gradient and projection pointers must be non-null");
+    CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= cbct_bp_workspace_bytes(A, D, H, W),
+                   "workspace too small for cbct_bp_backward");
+    hipStream_t stream = (hipStream_t)stream_;
+    ViewFrame* frames = (ViewFrame*)workspace;
+    const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    if ((st = launch_frames(gp, A, frames, stream))) return st;
+    if ((st = launch_fill(grad_projections, A * U * V, 0.0f, stream))) return st;
+    if ((st = launch_fill(grad_geometry, 8 * A, 0.0f, stream))) return st;
+    const BpGeom g = make_bp_geom(A, U, V, D, H, W, pixel_size, voxel_size, off_x, off_y, off_z);
+    const int K = bp_k(D);
+    dim3 grid((unsigned)ceil_div(H * W, BP_WAVES), (unsigned)ceil_div(D, 64 * K));
+    if (K == 4)
+        bp_backward_kernel<4><<<grid, 64 * BP_WAVES, 0, stream>>>(grad_volume, projections, frames, g,
+                                                                 grad_projections, grad_geometry);
+    else if (K == 2)
+        bp_backward_kernel<2><<<grid, 64 * BP_WAVES, 0, stream>>>(grad_volume, projections, frames, g,
+                                                                 grad_projections, grad_geometry);
+    else
+        bp_backward_kernel<1><<<grid, 64 * BP_WAVES, 0, stream>>>(grad_volume, projections, frames, g,
+                                                                 grad_projections, grad_geometry);
+    CBCT_LAUNCHED("backprojection backward kernel launch");
+    return CBCT_OK;
+}

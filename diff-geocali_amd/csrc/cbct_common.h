@@ -1,0 +1,247 @@
+// Shared device helpers for the cone-beam CT kernels (gfx950).
+//
+// All files are compiled with -ffp-contract=off: every fused multiply-add is
+// written out as fmaf(), so the float operation order below is the stated
+// FP32 reference schedule and is reproduced bit for bit by the CPU oracle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "cbct_hip.h"
+
+namespace cbct {
+
+// ---------------------------------------------------------------------------
+// error state (thread-local message, integer status)
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_status(hipError_t err, const char* what);
+
+#define CBCT_CHECK_ARG(cond, msg) \
+    do { if (!(cond)) return ::cbct::fail(CBCT_EINVAL, (msg)); } while (0)
+#define CBCT_HIP(call, what) \
+    do { hipError_t e_ = (call); if (e_ != hipSuccess) return ::cbct::hip_status(e_, (what)); } while (0)
+#define CBCT_LAUNCHED(what) \
+    do { hipError_t e_ = hipGetLastError(); if (e_ != hipSuccess) return ::cbct::hip_status(e_, (what)); } while (0)
+
+// ---------------------------------------------------------------------------
+// per-view frame
+// ---------------------------------------------------------------------------
+// Everything a view contributes to a ray or a voxel.  Trig is evaluated in
+// FP64 and rounded once; every other entry is the float expression of the
+// reference (csrc/src_cbct_forwardprojector_cuda.cu:133-151,
+// csrc/src_cbct_backprojector_cuda.cu:44-56).
+struct alignas(16) ViewFrame {
+    float c, s;          // cos / sin of the view angle
+    float ox, oy, oz;    // source position  (-sod*s + sx, sod*c + sy, sz)
+    float ns, nc;        // (sod - sid) * s, (sod - sid) * c
+    float u0, v0;        // detector centre indices
+    float sod, sid;
+    float dsu, dsn, dsv; // source offset in the rotating frame (backprojector)
+    float n;             // sod - sid
+    float pad;
+};
+static_assert(sizeof(ViewFrame) == 64, "ViewFrame is 16 floats");
+
+struct GeomPtrs {
+    const float* ang; const float* sid; const float* sod;
+    const float* u0; const float* v0;
+    const float* sx; const float* sy; const float* sz;
+};
+
+__device__ __forceinline__ ViewFrame make_frame(const GeomPtrs& g, int a) {
+    ViewFrame f;
+    const double beta = (double)g.ang[a];
+    f.c = (float)cos(beta);
+    f.s = (float)sin(beta);
+    const float sod = g.sod[a], sid = g.sid[a], sx = g.sx[a], sy = g.sy[a], sz = g.sz[a];
+    f.sod = sod;
+    f.sid = sid;
+    f.n = sod - sid;
+    f.ox = fmaf(-sod, f.s, sx);
+    f.oy = fmaf(sod, f.c, sy);
+    f.oz = sz;
+    f.ns = f.n * f.s;
+    f.nc = f.n * f.c;
+    f.u0 = g.u0[a];
+    f.v0 = g.v0[a];
+    f.dsu = fmaf(sx, f.c, sy * f.s);
+    f.dsn = fmaf(-sx, f.s, sy * f.c);
+    f.dsv = sz;
+    f.pad = 0.f;
+    return f;
+}
+
+// Axis-aligned volume box and voxel-coordinate map (x = W axis, y = H, z = D).
+// lo/hi: csrc/src_cbct_forwardprojector_cuda.cu:303-312.
+struct Box {
+    float lox, loy, loz, hix, hiy, hiz;
+    float offx, offy, offz;
+    float hx, hy, hz;   // W/2, H/2, D/2
+    int W, H, D;
+};
+
+inline Box make_box_host(int64_t D, int64_t H, int64_t W, float vs, float ox, float oy, float oz) {
+    Box b;
+    b.W = (int)W; b.H = (int)H; b.D = (int)D;
+    b.hx = (float)W * 0.5f; b.hy = (float)H * 0.5f; b.hz = (float)D * 0.5f;
+    b.lox = std::fmaf((float)(-W) * 0.5f - 0.5f, vs, ox);
+    b.loy = std::fmaf((float)(-H) * 0.5f - 0.5f, vs, oy);
+    b.loz = std::fmaf((float)(-D) * 0.5f - 0.5f, vs, oz);
+    b.hix = std::fmaf(b.hx - 0.5f, vs, ox);
+    b.hiy = std::fmaf(b.hy - 0.5f, vs, oy);
+    b.hiz = std::fmaf(b.hz - 0.5f, vs, oz);
+    b.offx = ox; b.offy = oy; b.offz = oz;
+    return b;
+}
+
+struct Ray {
+    float dx, dy, dz, len, up;
+};
+
+// Unit direction from the source through detector pixel (u,v); false for a
+// degenerate ray (csrc/src_cbct_forwardprojector_cuda.cu:138-169).
+__device__ __forceinline__ bool make_ray(const ViewFrame& f, int u, int v, float pix, Ray& r) {
+    const float up = ((float)u - f.u0) * pix;
+    const float vp = ((float)v - f.v0) * pix;
+    const float tx = fmaf(up, f.c, -f.ns);
+    const float ty = fmaf(up, f.s, f.nc);
+    const float rx = tx - f.ox, ry = ty - f.oy, rz = vp - f.oz;
+    const float len = sqrtf(fmaf(rz, rz, fmaf(ry, ry, rx * rx)));
+    r.up = up;
+    r.len = len;
+    if (len <= 1e-8f) return false;
+    const float inv = 1.0f / len;
+    r.dx = rx * inv; r.dy = ry * inv; r.dz = rz * inv;
+    return true;
+}
+
+// Slab update, csrc/src_cbct_forwardprojector_cuda.cu:55-83.
+__device__ __forceinline__ bool slab(float o, float d, float lo, float hi, int axis,
+                                     float& tmin, float& tmax, int& entry) {
+    if (fabsf(d) < 1e-12f) return o >= lo && o <= hi;
+    const float inv = 1.0f / d;
+    float t1 = (lo - o) * inv, t2 = (hi - o) * inv;
+    if (t1 > t2) { const float t = t1; t1 = t2; t2 = t; }
+    if (t1 > tmin) { tmin = t1; entry = axis; }
+    tmax = fminf(tmax, t2);
+    return tmax >= tmin;
+}
+
+// csrc/src_cbct_forwardprojector_cuda.cu:86-112.
+__device__ __forceinline__ bool intersect(const Box& b, const ViewFrame& f, const Ray& r,
+                                          float& tmin, float& tmax, int& entry) {
+    tmin = 0.0f; tmax = FLT_MAX; entry = -1;
+    if (!slab(f.ox, r.dx, b.lox, b.hix, 0, tmin, tmax, entry)) return false;
+    if (!slab(f.oy, r.dy, b.loy, b.hiy, 1, tmin, tmax, entry)) return false;
+    if (!slab(f.oz, r.dz, b.loz, b.hiz, 2, tmin, tmax, entry)) return false;
+    return tmax >= tmin;
+}
+
+// Voxel coordinate of a world coordinate.  POW2: voxel_size is a power of two,
+// so the division of the forward kernel (:326-328) equals the reciprocal
+// multiply of the adjoint (:425-427) and both are one fused operation.
+template <bool DIV>
+__device__ __forceinline__ float to_voxel(float p, float off, float vs, float inv_vs, float half) {
+    if (DIV) return (p - off) / vs + half;
+    return fmaf(p - off, inv_vs, half);
+}
+
+// Trilinear corner set of a sample: clamp-to-edge corner offsets and raw
+// fractions (csrc/src_cbct_forwardprojector_cuda.cu:186-229).
+struct Corners {
+    int i000, dx1, dy1, dz1;   // base offset and clamped +1 steps per axis
+    float fx, fy, fz;
+};
+
+__device__ __forceinline__ Corners corners(const Box& b, float x, float y, float z) {
+    const int xr = (int)floorf(x), yr = (int)floorf(y), zr = (int)floorf(z);
+    Corners c;
+    c.fx = x - (float)xr; c.fy = y - (float)yr; c.fz = z - (float)zr;
+    const int x0 = max(0, min(xr, b.W - 1)), x1 = max(0, min(xr + 1, b.W - 1));
+    const int y0 = max(0, min(yr, b.H - 1)), y1 = max(0, min(yr + 1, b.H - 1));
+    const int z0 = max(0, min(zr, b.D - 1)), z1 = max(0, min(zr + 1, b.D - 1));
+    const int P = b.H * b.W;
+    c.i000 = z0 * P + y0 * b.W + x0;
+    c.dx1 = x1 - x0;
+    c.dy1 = (y1 - y0) * b.W;
+    c.dz1 = (z1 - z0) * P;
+    return c;
+}
+
+struct Taps { float v000, v100, v010, v110, v001, v101, v011, v111; };
+
+__device__ __forceinline__ Taps load_taps(const float* __restrict__ vol, const Corners& c) {
+    const float* p = vol + c.i000;
+    Taps t;
+    t.v000 = p[0];               t.v100 = p[c.dx1];
+    t.v010 = p[c.dy1];           t.v110 = p[c.dy1 + c.dx1];
+    t.v001 = p[c.dz1];           t.v101 = p[c.dz1 + c.dx1];
+    t.v011 = p[c.dz1 + c.dy1];   t.v111 = p[c.dz1 + c.dy1 + c.dx1];
+    return t;
+}
+
+struct Weights { float w000, w100, w010, w110, w001, w101, w011, w111; };
+
+__device__ __forceinline__ Weights tri_weights(float fx, float fy, float fz) {
+    const float mx = 1.0f - fx, my = 1.0f - fy, mz = 1.0f - fz;
+    Weights w;
+    w.w000 = mx * my * mz; w.w100 = fx * my * mz; w.w010 = mx * fy * mz; w.w110 = fx * fy * mz;
+    w.w001 = mx * my * fz; w.w101 = fx * my * fz; w.w011 = mx * fy * fz; w.w111 = fx * fy * fz;
+    return w;
+}
+
+__device__ __forceinline__ float tri_value(const Taps& t, const Weights& w) {
+    float acc = t.v000 * w.w000;
+    acc = fmaf(t.v100, w.w100, acc); acc = fmaf(t.v010, w.w010, acc); acc = fmaf(t.v110, w.w110, acc);
+    acc = fmaf(t.v001, w.w001, acc); acc = fmaf(t.v101, w.w101, acc); acc = fmaf(t.v011, w.w011, acc);
+    acc = fmaf(t.v111, w.w111, acc);
+    return acc;
+}
+
+// Analytic spatial gradient of the trilinear interpolant in voxel units
+// (csrc/src_cbct_forwardprojector_cuda.cu:242-250).
+__device__ __forceinline__ void tri_gradient(const Taps& t, float fx, float fy, float fz,
+                                             float& gx, float& gy, float& gz) {
+    const float mx = 1.0f - fx, my = 1.0f - fy, mz = 1.0f - fz;
+    gx = fmaf(fz, fmaf(fy, t.v111 - t.v011, my * (t.v101 - t.v001)),
+              mz * fmaf(fy, t.v110 - t.v010, my * (t.v100 - t.v000)));
+    gy = fmaf(fz, fmaf(fx, t.v111 - t.v101, mx * (t.v011 - t.v001)),
+              mz * fmaf(fx, t.v110 - t.v100, mx * (t.v010 - t.v000)));
+    gz = fmaf(fy, fmaf(fx, t.v111 - t.v110, mx * (t.v011 - t.v010)),
+              my * fmaf(fx, t.v101 - t.v100, mx * (t.v001 - t.v000)));
+}
+
+// Wave-level (64 lanes) sum.
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
+    return x;
+}
+
+// Is x a power of two (so that division by it is exact scaling)?
+inline bool is_pow2(float x) {
+    int e = 0;
+    const float m = std::frexp(x, &e);
+    return m == 0.5f;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Checks shared by all entry points.
+int check_geometry(const float* const* g, int64_t A);
+int check_positive_finite(float x, const char* name);
+int check_finite(float x, const char* name);
+
+__global__ void frames_kernel(GeomPtrs g, int A, ViewFrame* __restrict__ out);
+__global__ void fill_kernel(float* __restrict__ p, int64_t n, float value);
+int launch_frames(const GeomPtrs& g, int64_t A, ViewFrame* out, hipStream_t stream);
+int launch_fill(float* p, int64_t n, float value, hipStream_t stream);
+
+}  // namespace cbct

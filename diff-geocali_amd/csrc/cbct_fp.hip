@@ -1,0 +1,343 @@
+// Forward ray-integral projector and its adjoint (volume gradient plus the
+// eight per-view geometry gradients) for gfx950.
+//
+// Ray assignment: a workgroup of 256 lanes owns a 16x16 tile of detector
+// pixels of one view; each wave owns an 8x8 sub-tile (8 consecutive v per
+// row, 8 rows of u).  Neighbouring lanes march neighbouring rays in lockstep,
+// so one wave-wide gather touches a compact 3-D neighbourhood of the volume
+// and mostly hits the vector L1 / L2 instead of HBM.
+//
+// Sample schedule (identical to the reference, csrc/src_cbct_forwardprojector_cuda.cu:321):
+// t starts at the box entry and is advanced by float addition of step_size
+// while t <= t_max.
+#include "cbct_common.h"
+
+#include <algorithm>
+
+namespace cbct {
+
+constexpr int FP_TILE = 16;          // rays per tile edge
+constexpr int FP_THREADS = FP_TILE * FP_TILE;
+
+struct TileMap {
+    int a, u, v;
+};
+
+// Block -> (view, tile) and lane -> ray.  Blocks are dispatched round-robin
+// over the 8 XCDs; consecutive tiles of one view are kept on one XCD so that
+// the rays sharing an L2 are neighbours.
+__device__ __forceinline__ TileMap tile_map(int tiles_u, int tiles_v, int nblocks) {
+    const int xcd = blockIdx.x & 7;
+    const int per_xcd = (nblocks + 7) >> 3;
+    int tile = xcd * per_xcd + (blockIdx.x >> 3);
+    if ((nblocks & 7) != 0) tile = blockIdx.x;   // uneven grid: plain order
+    const int tv = tile % tiles_v;
+    const int rest = tile / tiles_v;
+    const int tu = rest % tiles_u;
+    TileMap m;
+    m.a = rest / tiles_u;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    m.v = tv * FP_TILE + ((wave & 1) << 3) + (lane & 7);
+    m.u = tu * FP_TILE + ((wave >> 1) << 3) + (lane >> 3);
+    return m;
+}
+
+template <bool DIV>
+__global__ __launch_bounds__(FP_THREADS) void fp_forward_kernel(
+    const float* __restrict__ vol, Box b, const ViewFrame* __restrict__ frames,
+    int U, int V, int tiles_u, int tiles_v, int nblocks,
+    float pix, float step, float vs, float inv_vs, float* __restrict__ proj) {
+    const TileMap m = tile_map(tiles_u, tiles_v, nblocks);
+    if (m.u >= U || m.v >= V) return;
+    const ViewFrame f = frames[m.a];
+    float integral = 0.0f;
+    Ray ray;
+    if (make_ray(f, m.u, m.v, pix, ray)) {
+        float tmin, tmax;
+        int entry;
+        if (intersect(b, f, ray, tmin, tmax, entry)) {
+            for (float t = tmin; t <= tmax; t += step) {
+                const float px = fmaf(t, ray.dx, f.ox);
+                const float py = fmaf(t, ray.dy, f.oy);
+                const float pz = fmaf(t, ray.dz, f.oz);
+                const float x = to_voxel<DIV>(px, b.offx, vs, inv_vs, b.hx);
+                const float y = to_voxel<DIV>(py, b.offy, vs, inv_vs, b.hy);
+                const float z = to_voxel<DIV>(pz, b.offz, vs, inv_vs, b.hz);
+                const Corners c = corners(b, x, y, z);
+                const Taps tp = load_taps(vol, c);
+                integral += tri_value(tp, tri_weights(c.fx, c.fy, c.fz));
+            }
+            integral *= step;
+        }
+    }
+    proj[((int64_t)m.a * U + m.u) * V + m.v] = integral;
+}
+
+// Per-ray adjoint.  Writes one [8] partial of the geometry gradients per
+// workgroup (its tile); fp_reduce_geometry sums them in a fixed order.
+template <bool GVOL>
+__global__ __launch_bounds__(FP_THREADS) void fp_backward_kernel(
+    const float* __restrict__ grad, const float* __restrict__ vol, Box b,
+    const ViewFrame* __restrict__ frames, int U, int V, int tiles_u, int tiles_v, int nblocks,
+    float pix, float step, float inv_vs, float* __restrict__ gvol, float* __restrict__ partials) {
+    const TileMap m = tile_map(tiles_u, tiles_v, nblocks);
+    const bool active = m.u < U && m.v < V;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const float go = active ? grad[((int64_t)m.a * U + m.u) * V + m.v] : 0.0f;
+    const ViewFrame f = frames[m.a];
+    Ray ray;
+    if (active && go != 0.0f && make_ray(f, m.u, m.v, pix, ray)) {
+        float gox = 0.f, goy = 0.f, goz = 0.f, gdx = 0.f, gdy = 0.f, gdz = 0.f;
+        float tmin, tmax;
+        int entry;
+        if (intersect(b, f, ray, tmin, tmax, entry)) {
+            const float gs = go * step;
+            float gtmin = 0.0f;
+            for (float t = tmin; t <= tmax; t += step) {
+                const float px = fmaf(t, ray.dx, f.ox);
+                const float py = fmaf(t, ray.dy, f.oy);
+                const float pz = fmaf(t, ray.dz, f.oz);
+                const float x = fmaf(px - b.offx, inv_vs, b.hx);
+                const float y = fmaf(py - b.offy, inv_vs, b.hy);
+                const float z = fmaf(pz - b.offz, inv_vs, b.hz);
+                const Corners c = corners(b, x, y, z);
+                const Taps tp = load_taps(vol, c);
+                if (GVOL) {
+                    const Weights w = tri_weights(c.fx, c.fy, c.fz);
+                    float* p = gvol + c.i000;
+                    atomicAdd(p, gs * w.w000);
+                    atomicAdd(p + c.dx1, gs * w.w100);
+                    atomicAdd(p + c.dy1, gs * w.w010);
+                    atomicAdd(p + c.dy1 + c.dx1, gs * w.w110);
+                    atomicAdd(p + c.dz1, gs * w.w001);
+                    atomicAdd(p + c.dz1 + c.dx1, gs * w.w101);
+                    atomicAdd(p + c.dz1 + c.dy1, gs * w.w011);
+                    atomicAdd(p + c.dz1 + c.dy1 + c.dx1, gs * w.w111);
+                }
+                float sgx, sgy, sgz;
+                tri_gradient(tp, c.fx, c.fy, c.fz, sgx, sgy, sgz);
+                const float wx = sgx * inv_vs, wy = sgy * inv_vs, wz = sgz * inv_vs;
+                gox = fmaf(gs, wx, gox); goy = fmaf(gs, wy, goy); goz = fmaf(gs, wz, goz);
+                gdx = fmaf(gs * wx, t, gdx); gdy = fmaf(gs * wy, t, gdy); gdz = fmaf(gs * wz, t, gdz);
+                gtmin = fmaf(gs, fmaf(wz, ray.dz, fmaf(wy, ray.dy, wx * ray.dx)), gtmin);
+            }
+            // The samples are anchored at the entry face (:454-467).
+            if (entry == 0) { gox -= gtmin / ray.dx; gdx -= gtmin * tmin / ray.dx; }
+            else if (entry == 1) { goy -= gtmin / ray.dy; gdy -= gtmin * tmin / ray.dy; }
+            else if (entry == 2) { goz -= gtmin / ray.dz; gdz -= gtmin * tmin / ray.dz; }
+        }
+        // d = (T - O)/|T - O|  (:470-486)
+        const float dd = fmaf(ray.dz, gdz, fmaf(ray.dy, gdy, ray.dx * gdx));
+        const float il = 1.0f / ray.len;
+        const float grx = fmaf(-ray.dx, dd, gdx) * il;
+        const float gry = fmaf(-ray.dy, dd, gdy) * il;
+        const float grz = fmaf(-ray.dz, dd, gdz) * il;
+        const float Ox = gox - grx, Oy = goy - gry, Oz = goz - grz;
+        // per-view parameters (:495-527)
+        const float dOxb = -f.sod * f.c, dOyb = -f.sod * f.s;
+        const float dTxb = fmaf(-ray.up, f.s, -f.nc);
+        const float dTyb = fmaf(ray.up, f.c, -f.ns);
+        acc[0] = fmaf(gry, dTyb, fmaf(grx, dTxb, fmaf(Oy, dOyb, Ox * dOxb)));
+        acc[1] = fmaf(grx, f.s, -(gry * f.c));
+        acc[2] = fmaf(Oy + gry, f.c, (Ox + grx) * -f.s);
+        acc[3] = -pix * fmaf(gry, f.s, grx * f.c);
+        acc[4] = -pix * grz;
+        acc[5] = Ox; acc[6] = Oy; acc[7] = Oz;
+    }
+    __shared__ float red[FP_THREADS / 64][8];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float s = wave_sum(acc[k]);
+        if (lane == 0) red[wave][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int k = threadIdx.x;
+        const float s = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+        const int tile_in_view = (int)(((int64_t)m.u / FP_TILE) * tiles_v + m.v / FP_TILE);
+        partials[((int64_t)m.a * tiles_u * tiles_v + tile_in_view) * 8 + k] = s;
+    }
+}
+
+// Sum the per-tile partials of each view in a fixed order: ggeom[k][a].
+__global__ __launch_bounds__(256) void fp_reduce_geometry(const float* __restrict__ partials,
+                                                          int A, int tiles, float* __restrict__ ggeom) {
+    const int a = blockIdx.x;
+    __shared__ float sm[256][9];
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int t = threadIdx.x; t < tiles; t += 256) {
+        const float* p = partials + ((int64_t)a * tiles + t) * 8;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] += p[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sm[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sm[threadIdx.x][k] += sm[threadIdx.x + s][k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) ggeom[(int64_t)threadIdx.x * A + a] = sm[0][threadIdx.x];
+}
+
+__global__ __launch_bounds__(FP_THREADS) void fp_count_kernel(
+    Box b, const ViewFrame* __restrict__ frames, int U, int V, int tiles_u, int tiles_v, int nblocks,
+    float pix, float step, unsigned long long* __restrict__ count) {
+    const TileMap m = tile_map(tiles_u, tiles_v, nblocks);
+    unsigned long long n = 0;
+    if (m.u < U && m.v < V) {
+        const ViewFrame f = frames[m.a];
+        Ray ray;
+        float tmin, tmax;
+        int entry;
+        if (make_ray(f, m.u, m.v, pix, ray) && intersect(b, f, ray, tmin, tmax, entry))
+            for (float t = tmin; t <= tmax; t += step) ++n;
+    }
+    __shared__ unsigned long long red[FP_THREADS / 64];
+    for (int s = 32; s >= 1; s >>= 1) n += __shfl_xor(n, s, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(count, (red[0] + red[1]) + (red[2] + red[3]));
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+static int64_t fp_tiles(int64_t U, int64_t V) { return ceil_div(U, FP_TILE) * ceil_div(V, FP_TILE); }
+
+static int check_fp_common(int64_t D, int64_t H, int64_t W, int64_t A, int64_t U, int64_t V,
+                           float pix, float step, float vs, float ox, float oy, float oz) {
+    CBCT_CHECK_ARG(D > 0 && H > 0 && W > 0, "every volume dimension must be positive");
+    CBCT_CHECK_ARG(U > 0 && V > 0, "U and V must be positive");
+    CBCT_CHECK_ARG(A > 0, "angles must contain at least one view");
+    CBCT_CHECK_ARG(D * H * W <= INT32_MAX, "volume exceeds the 2^31-1 element limit");
+    CBCT_CHECK_ARG(A * fp_tiles(U, V) <= INT32_MAX, "too many rays for one launch");
+    int st;
+    if ((st = check_positive_finite(pix, "pixel_size"))) return st;
+    if ((st = check_positive_finite(step, "step_size"))) return st;
+    if ((st = check_positive_finite(vs, "voxel_size"))) return st;
+    if ((st = check_finite(ox, "off_x"))) return st;
+    if ((st = check_finite(oy, "off_y"))) return st;
+    if ((st = check_finite(oz, "off_z"))) return st;
+    return CBCT_OK;
+}
+
+}  // namespace cbct
+
+using namespace cbct;
+
+extern "C" size_t cbct_fp_workspace_bytes(int64_t A, int64_t U, int64_t V) {
+    if (A < 1 || U < 1 || V < 1) return 0;
+    return align256((size_t)A * sizeof(ViewFrame)) + (size_t)A * fp_tiles(U, V) * 8 * sizeof(float);
+}
+
+extern "C" int cbct_fp_forward(const float* volume, int64_t D, int64_t H, int64_t W,
+                               const float* angles, const float* sid, const float* sod,
+                               const float* det_u0, const float* det_v0,
+                               const float* src_x, const float* src_y, const float* src_z, int64_t A,
+                               float pixel_size, int64_t U, int64_t V, float step_size,
+                               float voxel_size, float off_x, float off_y, float off_z,
+                               float* projections, void* workspace, size_t workspace_bytes,
+                               void* stream_) {
+    int st;
+    if ((st = check_fp_common(D, H, W, A, U, V, pixel_size, step_size, voxel_size, off_x, off_y, off_z))) return st;
+    const float* g[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    if ((st = check_geometry(g, A))) return st;
+    CBCT_CHECK_ARG(volume != nullptr && projections != nullptr, "volume and projections must be non-null");
+    CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= cbct_fp_workspace_bytes(A, U, V),
+                   "workspace too small for cbct_fp_forward");
+    hipStream_t stream = (hipStream_t)stream_;
+    ViewFrame* frames = (ViewFrame*)workspace;
+    const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    if ((st = launch_frames(gp, A, frames, stream))) return st;
+    const Box b = make_box_host(D, H, W, voxel_size, off_x, off_y, off_z);
+    const int tu = (int)ceil_div(U, FP_TILE), tv = (int)ceil_div(V, FP_TILE);
+    const int nblocks = (int)(A * tu * tv);
+    const float inv_vs = 1.0f / voxel_size;
+    if (is_pow2(voxel_size))
+        fp_forward_kernel<false><<<nblocks, FP_THREADS, 0, stream>>>(
+            volume, b, frames, (int)U, (int)V, tu, tv, nblocks, pixel_size, step_size, voxel_size,
+            inv_vs, projections);
+    else
+        fp_forward_kernel<true><<<nblocks, FP_THREADS, 0, stream>>>(
+            volume, b, frames, (int)U, (int)V, tu, tv, nblocks, pixel_size, step_size, voxel_size,
+            inv_vs, projections);
+    CBCT_LAUNCHED("source-only forward kernel launch");
+    return CBCT_OK;
+}
+
+extern "C" int cbct_fp_backward(const float* grad_projection, const float* volume,
+                                int64_t D, int64_t H, int64_t W,
+                                const float* angles, const float* sid, const float* sod,
+                                const float* det_u0, const float* det_v0,
+                                const float* src_x, const float* src_y, const float* src_z,
+                                int64_t A, int64_t U, int64_t V,
+                                float pixel_size, float step_size, float voxel_size,
+                                int compute_grad_volume, float off_x, float off_y, float off_z,
+                                float* grad_volume, float* grad_geometry,
+                                void* workspace, size_t workspace_bytes, void* stream_) {
+    int st;
+    if ((st = check_fp_common(D, H, W, A, U, V, pixel_size, step_size, voxel_size, off_x, off_y, off_z))) return st;
+    const float* g[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    if ((st = check_geometry(g, A))) return st;
+    CBCT_CHECK_ARG(grad_projection != nullptr && volume != nullptr && grad_geometry != nullptr,
+                   "grad_projection, volume and grad_geometry must be non-null");
+    CBCT_CHECK_ARG(!compute_grad_volume || grad_volume != nullptr,
+                   "grad_volume must be non-null when compute_grad_volume is set");
+    CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= cbct_fp_workspace_bytes(A, U, V),
+                   "workspace too small for cbct_fp_backward");
+    hipStream_t stream = (hipStream_t)stream_;
+    ViewFrame* frames = (ViewFrame*)workspace;
+    float* partials = (float*)((char*)workspace + align256((size_t)A * sizeof(ViewFrame)));
+    const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    if ((st = launch_frames(gp, A, frames, stream))) return st;
+    if (compute_grad_volume && (st = launch_fill(grad_volume, D * H * W, 0.0f, stream))) return st;
+    const Box b = make_box_host(D, H, W, voxel_size, off_x, off_y, off_z);
+    const int tu = (int)ceil_div(U, FP_TILE), tv = (int)ceil_div(V, FP_TILE);
+    const int nblocks = (int)(A * tu * tv);
+    const float inv_vs = 1.0f / voxel_size;
+    if (compute_grad_volume)
+        fp_backward_kernel<true><<<nblocks, FP_THREADS, 0, stream>>>(
+            grad_projection, volume, b, frames, (int)U, (int)V, tu, tv, nblocks, pixel_size,
+            step_size, inv_vs, grad_volume, partials);
+    else
+        fp_backward_kernel<false><<<nblocks, FP_THREADS, 0, stream>>>(
+            grad_projection, volume, b, frames, (int)U, (int)V, tu, tv, nblocks, pixel_size,
+            step_size, inv_vs, nullptr, partials);
+    CBCT_LAUNCHED("source-only backward kernel launch");
+    fp_reduce_geometry<<<(unsigned)A, 256, 0, stream>>>(partials, (int)A, tu * tv, grad_geometry);
+    CBCT_LAUNCHED("geometry reduction kernel launch");
+    return CBCT_OK;
+}
+
+extern "C" int cbct_fp_count_samples(int64_t D, int64_t H, int64_t W,
+                                     const float* angles, const float* sid, const float* sod,
+                                     const float* det_u0, const float* det_v0,
+                                     const float* src_x, const float* src_y, const float* src_z, int64_t A,
+                                     float pixel_size, int64_t U, int64_t V, float step_size,
+                                     float voxel_size, float off_x, float off_y, float off_z,
+                                     uint64_t* count, void* workspace, size_t workspace_bytes,
+                                     void* stream_) {
+    int st;
+    if ((st = check_fp_common(D, H, W, A, U, V, pixel_size, step_size, voxel_size, off_x, off_y, off_z))) return st;
+    const float* g[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    if ((st = check_geometry(g, A))) return st;
+    CBCT_CHECK_ARG(count != nullptr, "count must be non-null");
+    CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= cbct_fp_workspace_bytes(A, U, V),
+                   "workspace too small for cbct_fp_count_samples");
+    hipStream_t stream = (hipStream_t)stream_;
+    ViewFrame* frames = (ViewFrame*)workspace;
+    const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    if ((st = launch_frames(gp, A, frames, stream))) return st;
+    const Box b = make_box_host(D, H, W, voxel_size, off_x, off_y, off_z);
+    const int tu = (int)ceil_div(U, FP_TILE), tv = (int)ceil_div(V, FP_TILE);
+    const int nblocks = (int)(A * tu * tv);
+    fp_count_kernel<<<nblocks, FP_THREADS, 0, stream>>>(b, frames, (int)U, (int)V, tu, tv, nblocks,
+                                                        pixel_size, step_size,
+                                                        (unsigned long long*)count);
+    CBCT_LAUNCHED("sample count kernel launch");
+    return CBCT_OK;
+}

@@ -1,0 +1,145 @@
+/*
+ * cbct_hip.h -- C ABI of the MI355X (gfx950) cone-beam CT operator library
+ * libcbct_hip.so.
+ *
+ * Every entry point takes raw device pointers, int64 sizes, float scalars, a
+ * caller-owned device workspace and a hipStream_t passed as void*.  Calls are
+ * asynchronous on that stream and never synchronise; the caller makes the
+ * stream's device current.  Outputs are allocated by the caller and fully
+ * overwritten.  Return value: 0 on success, otherwise a CBCT_E* code with a
+ * message available from cbct_last_error() (thread-local).
+ *
+ * Geometry is always passed as eight device arrays of A floats, in the order
+ * angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z (radians, total SDD,
+ * SOD, detector-centre pixel indices, world-coordinate source offsets).
+ * Geometry gradients are written to one [8][A] float array in that order.
+ *
+ * Citations name the reference interface each entry point replaces (paths
+ * relative to the reference repository).
+ */
+#ifndef CBCT_HIP_H
+#define CBCT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CBCT_OK = 0,
+    CBCT_EINVAL = 1,   /* invalid argument (shape, scalar, null pointer, limit) */
+    CBCT_EWORKSPACE = 2, /* workspace too small */
+    CBCT_EHIP = 3      /* HIP runtime error (launch or device) */
+};
+
+/* Library version string, e.g. "cbct_hip 0.3.0 gfx950". */
+const char* cbct_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* cbct_last_error(void);
+
+/* ---- forward projector: src_cbct_forwardprojector_cuda ----------------- */
+
+/* Workspace bytes needed by cbct_fp_forward / cbct_fp_backward. */
+size_t cbct_fp_workspace_bytes(int64_t A, int64_t U, int64_t V);
+
+/* Fixed-step trilinear ray integrals proj[A,U,V] of volume[D,H,W].
+ * Replaces forward_cuda, csrc/src_cbct_forwardprojector_cuda.cu:566-631
+ * (pybind "forward", :733-744). */
+int cbct_fp_forward(const float* volume, int64_t D, int64_t H, int64_t W,
+                    const float* angles, const float* sid, const float* sod,
+                    const float* det_u0, const float* det_v0,
+                    const float* src_x, const float* src_y, const float* src_z, int64_t A,
+                    float pixel_size, int64_t U, int64_t V, float step_size, float voxel_size,
+                    float off_x, float off_y, float off_z,
+                    float* projections, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Adjoint of cbct_fp_forward for the upstream gradient grad_projection[A,U,V]:
+ * grad_volume[D,H,W] (skipped and may be NULL when compute_grad_volume == 0)
+ * and grad_geometry[8][A].
+ * Replaces backward_cuda, csrc/src_cbct_forwardprojector_cuda.cu:634-729
+ * (pybind "backward", :746-757). */
+int cbct_fp_backward(const float* grad_projection, const float* volume,
+                     int64_t D, int64_t H, int64_t W,
+                     const float* angles, const float* sid, const float* sod,
+                     const float* det_u0, const float* det_v0,
+                     const float* src_x, const float* src_y, const float* src_z,
+                     int64_t A, int64_t U, int64_t V,
+                     float pixel_size, float step_size, float voxel_size,
+                     int compute_grad_volume, float off_x, float off_y, float off_z,
+                     float* grad_volume, float* grad_geometry,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* Exact number of trilinear samples of the forward schedule (the sum over
+ * rays of the loop trip counts of cbct_fp_forward), accumulated into
+ * count[0] (a device uint64 the caller zeroes).  No reference counterpart:
+ * it is the unit of the projectors' algorithmic traffic. */
+int cbct_fp_count_samples(int64_t D, int64_t H, int64_t W,
+                          const float* angles, const float* sid, const float* sod,
+                          const float* det_u0, const float* det_v0,
+                          const float* src_x, const float* src_y, const float* src_z, int64_t A,
+                          float pixel_size, int64_t U, int64_t V, float step_size, float voxel_size,
+                          float off_x, float off_y, float off_z, uint64_t* count,
+                          void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- weighted backprojector: src_cbct_backprojector_cuda --------------- */
+
+/* Workspace bytes needed by cbct_bp_forward / cbct_bp_backward. */
+size_t cbct_bp_workspace_bytes(int64_t A, int64_t D, int64_t H, int64_t W);
+
+/* volume[D,H,W] = sum over views of FDK-weighted bilinear samples of
+ * projections[A,U,V].  Replaces forward_cuda,
+ * csrc/src_cbct_backprojector_cuda.cu:344-420 (pybind "forward", :525-532). */
+int cbct_bp_forward(const float* projections, int64_t A, int64_t U, int64_t V,
+                    const float* angles, const float* sid, const float* sod,
+                    const float* det_u0, const float* det_v0,
+                    const float* src_x, const float* src_y, const float* src_z,
+                    float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
+                    float off_x, float off_y, float off_z,
+                    float* volume, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Backward of cbct_bp_forward for grad_volume[D,H,W]: grad_projections[A,U,V]
+ * and grad_geometry[8][A].  Replaces backward_cuda,
+ * csrc/src_cbct_backprojector_cuda.cu:423-522 (pybind "backward", :534-542). */
+int cbct_bp_backward(const float* grad_volume, const float* projections,
+                     int64_t A, int64_t U, int64_t V,
+                     const float* angles, const float* sid, const float* sod,
+                     const float* det_u0, const float* det_v0,
+                     const float* src_x, const float* src_y, const float* src_z,
+                     float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
+                     float off_x, float off_y, float off_z,
+                     float* grad_projections, float* grad_geometry,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- FDK ---------------------------------------------------------------- */
+
+/* Workspace bytes needed by cbct_fdk_filter (filter only) and
+ * cbct_fdk_reconstruct (filter + backprojection). */
+size_t cbct_fdk_workspace_bytes(int64_t A, int64_t U, int64_t V);
+
+/* Centred cosine preweight, Hann-windowed discrete ramp filter along U and the
+ * pixel_size*sid/sod scale; sod and sid are device arrays of A values.
+ * Replaces fdk_filter, cbct_backprojector/filtering.py:54-104. */
+int cbct_fdk_filter(const float* projections, int64_t A, int64_t U, int64_t V,
+                    float pixel_size, const float* sod, const float* sid, float alpha,
+                    float* filtered, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Filter, backproject in view chunks of angle_chunk_size (each chunk summed
+ * from zero, then added), scale by pi/A and clamp at zero.
+ * Replaces reconstruct_fdk, cbct_backprojector/filtering.py:107-198. */
+int cbct_fdk_reconstruct(const float* projections, int64_t A, int64_t U, int64_t V,
+                         const float* angles, const float* sid, const float* sod,
+                         const float* det_u0, const float* det_v0,
+                         const float* src_x, const float* src_y, const float* src_z,
+                         float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
+                         float off_x, float off_y, float off_z,
+                         float alpha, int64_t angle_chunk_size,
+                         float* volume, void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CBCT_HIP_H */

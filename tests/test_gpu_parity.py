@@ -1,0 +1,279 @@
+"""Parity of the HIP operators with the CPU oracle (GPU).
+
+Bars: the forward projection must match the FP32 reference schedule to 1e-6
+relative L2 (it follows the same float operation order) and the FP64 oracle
+to the reference's tolerance 5e-5 (examples/check_forward_operator.py:220);
+adjoint / backprojection gradients are compared with the FP64 oracle at the
+reference's tolerances (check_forward_operator.py:226, check_operator.py:199-203)
+and, for the volume gradient, by relative L2.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+import oracle
+from cbct_backprojector import backproject, detector_centers, fdk_filter, project, reconstruct_fdk
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def _gpu(t, grad=True):
+    return t.detach().to(DEV).requires_grad_(grad)
+
+
+# ------------------------------------------------------------ reference fixtures
+
+def test_fp_fixture_forward_and_gradients():
+    """check_forward_operator.check_cuda: non-contiguous inputs through project()."""
+    vol, geom, up = cases.fp_case(torch.float32)
+    params = [_gpu(vol)] + [_gpu(g) for g in geom]
+    assert not params[0].is_contiguous()
+    gup = up.to(DEV)
+    assert not gup.is_contiguous()
+    out = project(*params, cases.FP_PIXEL, cases.FP_U, cases.FP_V, step_size=cases.FP_STEP,
+                  voxel_size=cases.FP_VOXEL, off_x=cases.FP_OFFSET[0], off_y=cases.FP_OFFSET[1],
+                  off_z=cases.FP_OFFSET[2])
+    grads = torch.autograd.grad(out, params, grad_outputs=gup)
+    torch.cuda.synchronize()
+    vnp = vol.contiguous().numpy()
+    gnp = [g.numpy() for g in geom]
+    args = (cases.FP_PIXEL, cases.FP_U, cases.FP_V, cases.FP_STEP, cases.FP_VOXEL, cases.FP_OFFSET)
+    ref32 = oracle.fp_forward(vnp, gnp, *args, mode="f32")
+    ref64 = oracle.fp_forward(vnp.astype(np.float64), [g.astype(np.float64) for g in gnp], *args, mode="f64")
+    got = cases.to_np(out)
+    assert cases.rel_l2(got, ref32) <= 1e-6
+    np.testing.assert_allclose(got, ref64, rtol=5e-5, atol=5e-5)
+    golden = np.array(cases.load_golden()["fp_case"]["expected_projection_f64"])
+    np.testing.assert_allclose(got, golden, rtol=5e-5, atol=5e-5)
+    gvol, ggeo = oracle.fp_backward(up.contiguous().numpy().astype(np.float64), vnp.astype(np.float64),
+                                    [g.astype(np.float64) for g in gnp], cases.FP_PIXEL, cases.FP_STEP,
+                                    cases.FP_VOXEL, cases.FP_OFFSET, mode="f64")
+    np.testing.assert_allclose(cases.to_np(grads[0]), gvol, rtol=1e-3, atol=1e-4)
+    for k in range(8):
+        np.testing.assert_allclose(cases.to_np(grads[k + 1]), ggeo[k], rtol=1e-3, atol=1e-4)
+
+
+def test_fp_fixture_linearity_and_local_finite_differences():
+    """check_forward_operator.check_cuda:230-259 -- FP32 FD with eps 1e-3, traces from
+    the FP64 oracle on the float32-perturbed inputs."""
+    vol, geom, _ = cases.fp_case(torch.float32)
+    params = [_gpu(vol)] + [_gpu(g) for g in geom]
+
+    def run(values):
+        return project(*values, cases.FP_PIXEL, cases.FP_U, cases.FP_V, step_size=cases.FP_STEP,
+                       voxel_size=cases.FP_VOXEL, off_x=cases.FP_OFFSET[0], off_y=cases.FP_OFFSET[1],
+                       off_z=cases.FP_OFFSET[2])
+
+    with torch.no_grad():
+        base = run(params)
+        other = params[0].flip(-1)
+        comb = run([1.3 * params[0] - 0.4 * other, *params[1:]])
+        torch.testing.assert_close(comb, 1.3 * base - 0.4 * run([other, *params[1:]]), rtol=5e-5, atol=5e-5)
+    sel = cases.FP_SELECTED_RAY
+    out = run(params)
+    sg = torch.autograd.grad(cases.FP_SELECTED_WEIGHT * out[sel], params)
+    args = (cases.FP_PIXEL, cases.FP_U, cases.FP_V, cases.FP_STEP, cases.FP_VOXEL, cases.FP_OFFSET)
+
+    def trace(values):
+        v = [x.detach().cpu().to(torch.float64).contiguous().numpy() for x in values]
+        _, tr = oracle.fp_forward(v[0], v[1:], *args, mode="f64", traces=True)
+        return tuple(t[sel] for t in tr)
+
+    base_trace = trace(params)
+    eps = 1e-3
+    idx = np.unravel_index(int(sg[0].abs().argmax()), tuple(sg[0].shape))
+    checks = [(0, idx)] + [(k, (sel[0],)) for k in (7, 3, 4)]
+    for pos, index in checks:
+        plus = [x.detach().clone() for x in params]
+        minus = [x.detach().clone() for x in params]
+        plus[pos][index] += eps
+        minus[pos][index] -= eps
+        assert trace(plus) == base_trace and trace(minus) == base_trace
+        with torch.no_grad():
+            diff = (run(plus)[sel] - run(minus)[sel]).item()
+        fd = cases.FP_SELECTED_WEIGHT * diff / (plus[pos][index] - minus[pos][index]).item()
+        assert math.isclose(sg[pos][index].item(), fd, rel_tol=1e-2, abs_tol=2e-3), (pos, sg[pos][index], fd)
+
+
+def test_bp_fixture_forward_and_gradients():
+    """check_operator.check_cuda: forward 2e-5, nine gradients rtol 3e-4 / atol 3e-5."""
+    proj, geom, up = cases.bp_case(torch.float32)
+    params = [_gpu(proj)] + [_gpu(g) for g in geom]
+    gup = up.to(DEV)
+    out = backproject(*params, cases.BP_PIXEL, cases.BP_SHAPE, cases.BP_VOXEL, volume_offset=cases.BP_OFFSET)
+    grads = torch.autograd.grad(out, params, grad_outputs=gup)
+    torch.cuda.synchronize()
+    pnp = proj.contiguous().numpy().astype(np.float64)
+    gnp = [g.numpy().astype(np.float64) for g in geom]
+    ref = oracle.bp_forward(pnp, gnp, cases.BP_PIXEL, cases.BP_SHAPE, cases.BP_VOXEL, cases.BP_OFFSET, mode="f64")
+    np.testing.assert_allclose(cases.to_np(out), ref, rtol=2e-5, atol=2e-5)
+    golden = np.array(cases.load_golden()["bp_case"]["expected_volume_f64"])
+    np.testing.assert_allclose(cases.to_np(out), golden, rtol=2e-5, atol=2e-5)
+    gproj, ggeo = oracle.bp_backward(up.contiguous().numpy().astype(np.float64), pnp, gnp, cases.BP_PIXEL,
+                                     cases.BP_VOXEL, cases.BP_OFFSET, mode="f64")
+    np.testing.assert_allclose(cases.to_np(grads[0]), gproj, rtol=3e-4, atol=3e-5)
+    for k in range(8):
+        np.testing.assert_allclose(cases.to_np(grads[k + 1]), ggeo[k], rtol=3e-4, atol=3e-5)
+
+
+# ------------------------------------------------------------ CPU-runnable reference size
+
+def c1_tensors():
+    vol = cases.c1_volume()
+    geom = cases.c1_geometry()
+    return vol, geom
+
+
+def test_c1_pipeline_project_backproject_backward():
+    """project -> backproject -> backward of mean(proj^2) w.r.t. volume and src_x."""
+    c = cases.C1
+    vol, geom = c1_tensors()
+    tv = torch.tensor(vol, device=DEV, requires_grad=True)
+    tg = [torch.tensor(g, device=DEV) for g in geom]
+    tg[5].requires_grad_(True)
+    proj = project(tv, *tg, c["pixel"], c["U"], c["V"], step_size=c["step"], voxel_size=c["voxel"])
+    loss = proj.square().mean()
+    loss.backward()
+    bp = backproject(proj.detach(), *[t.detach() for t in tg], c["pixel"], c["shape"], c["voxel"])
+    torch.cuda.synchronize()
+    args = (c["pixel"], c["U"], c["V"], c["step"], c["voxel"])
+    ref32 = oracle.fp_forward(vol, geom, *args, mode="f32")
+    ref64 = oracle.fp_forward(vol.astype(np.float64), geom, *args, mode="f64")
+    got = cases.to_np(proj)
+    e32, e64 = cases.rel_l2(got, ref32), cases.rel_l2(got, ref64)
+    print(f"c1 forward rel L2: vs FP32 schedule {e32:.3e}, vs FP64 {e64:.3e}")
+    assert e32 <= 1e-6 and e64 <= 1e-5
+    upstream = 2.0 * ref64 / ref64.size
+    gvol, ggeo = oracle.fp_backward(upstream, vol.astype(np.float64), geom, c["pixel"], c["step"], c["voxel"],
+                                    mode="f64")
+    assert cases.rel_l2(cases.to_np(tv.grad), gvol) <= 1e-5
+    assert cases.rel_l2(cases.to_np(tg[5].grad), ggeo[5]) <= 1e-4
+    bref = oracle.bp_forward(got.astype(np.float64), geom, c["pixel"], c["shape"], c["voxel"], mode="f64")
+    assert cases.rel_l2(cases.to_np(bp), bref) <= 1e-5
+
+
+def test_c1_all_nine_gradients_and_native_modules():
+    c = cases.C1
+    vol, geom = c1_tensors()
+    tv = torch.tensor(vol, device=DEV)
+    tg = [torch.tensor(g, device=DEV) for g in geom]
+    import src_cbct_backprojector_cuda as bpn
+    import src_cbct_forwardprojector_cuda as fpn
+    proj = fpn.forward(tv, *tg, c["pixel"], c["U"], c["V"], c["step"], c["voxel"])
+    rng = np.random.default_rng(4)
+    up = rng.standard_normal(proj.shape).astype(np.float32)
+    grads = fpn.backward(torch.tensor(up, device=DEV), tv, *tg, c["pixel"], c["step"], c["voxel"])
+    g_nov = fpn.backward(torch.tensor(up, device=DEV), tv, *tg, c["pixel"], c["step"], c["voxel"],
+                         compute_grad_volume=False)
+    assert g_nov[0].numel() == 0
+    gvol, ggeo = oracle.fp_backward(up.astype(np.float64), vol.astype(np.float64), geom, c["pixel"], c["step"],
+                                    c["voxel"], mode="f64")
+    assert cases.rel_l2(cases.to_np(grads[0]), gvol) <= 1e-5
+    for k in range(8):
+        assert cases.rel_l2(cases.to_np(grads[k + 1]), ggeo[k]) <= 1e-4, k
+        np.testing.assert_array_equal(cases.to_np(g_nov[k + 1]), cases.to_np(grads[k + 1]))
+    vb = bpn.forward(proj, *tg, c["pixel"], *c["shape"], c["voxel"])
+    gv = rng.standard_normal(vb.shape).astype(np.float32)
+    bg = bpn.backward(torch.tensor(gv, device=DEV), proj, *tg, c["pixel"], c["voxel"])
+    gproj, bgeo = oracle.bp_backward(gv.astype(np.float64), cases.to_np(proj).astype(np.float64), geom,
+                                     c["pixel"], c["voxel"], mode="f64")
+    assert cases.rel_l2(cases.to_np(bg[0]), gproj) <= 1e-5
+    for k in range(8):
+        assert cases.rel_l2(cases.to_np(bg[k + 1]), bgeo[k]) <= 1e-4, k
+
+
+def test_adjoint_dot_product_midsize():
+    """<P x, y> = <x, P^T y> with the GPU forward and adjoint (64^3, 30 views)."""
+    D = H = W = 64
+    A, U, V = 30, 96, 80
+    geom = [np.arange(A, dtype=np.float32) * np.float32(2 * math.pi / A), np.full(A, 300.0, np.float32),
+            np.full(A, 200.0, np.float32), np.full(A, 47.5, np.float32), np.full(A, 39.5, np.float32),
+            np.full(A, 0.3, np.float32), np.full(A, -0.2, np.float32), np.full(A, 0.1, np.float32)]
+    rng = np.random.default_rng(9)
+    x = torch.tensor(rng.random((D, H, W), dtype=np.float32), device=DEV, requires_grad=True)
+    y = torch.tensor(rng.standard_normal((A, U, V)).astype(np.float32), device=DEV)
+    tg = [torch.tensor(g, device=DEV) for g in geom]
+    px = project(x, *tg, 0.9, U, V, step_size=0.25, voxel_size=0.5)
+    (gx,) = torch.autograd.grad(px, [x], grad_outputs=y)
+    lhs = (px.double() * y.double()).sum().item()
+    rhs = (gx.double() * x.detach().double()).sum().item()
+    assert math.isclose(lhs, rhs, rel_tol=1e-5), (lhs, rhs)
+
+
+# ------------------------------------------------------------ FDK
+
+def _fdk_case(A=48, U=64, V=40, shape=(32, 36, 40)):
+    rng = np.random.default_rng(6)
+    proj = rng.random((A, U, V)).astype(np.float32)
+    geom = [np.arange(A, dtype=np.float32) * np.float32(2 * math.pi / A), np.full(A, 200.0, np.float32),
+            np.full(A, 120.0, np.float32), np.full(A, (U - 1) / 2 + 0.3, np.float32),
+            np.full(A, (V - 1) / 2 - 0.2, np.float32), np.full(A, 0.2, np.float32),
+            np.full(A, -0.1, np.float32), np.full(A, 0.05, np.float32)]
+    return proj, geom, shape
+
+
+def test_fdk_filter_matches_fp64_fft_oracle():
+    proj, geom, _ = _fdk_case()
+    for U, alpha in ((64, 0.8), (37, 1.0)):
+        p = proj[:, :U, :]
+        out = fdk_filter(torch.tensor(p, device=DEV), 0.8, torch.tensor(geom[2], device=DEV),
+                         torch.tensor(geom[1], device=DEV), alpha=alpha)
+        ref = oracle.fdk_filter(p.astype(np.float64), 0.8, geom[2], geom[1], alpha)
+        err = cases.rel_l2(cases.to_np(out), ref)
+        print(f"fdk_filter U={U}: rel L2 vs FP64 {err:.3e}")
+        assert err <= 1e-5
+    # scalar distances broadcast like the reference
+    out_s = fdk_filter(torch.tensor(proj, device=DEV), 0.8, 120.0, 200.0)
+    ref_s = oracle.fdk_filter(proj.astype(np.float64), 0.8, 120.0, 200.0, 0.8)
+    assert cases.rel_l2(cases.to_np(out_s), ref_s) <= 1e-5
+
+
+def test_reconstruct_fdk_matches_oracle_and_chunks():
+    proj, geom, shape = _fdk_case()
+    tp = torch.tensor(proj, device=DEV)
+    tg = [torch.tensor(g, device=DEV) for g in geom]
+    vol = reconstruct_fdk(tp, *tg, 0.8, shape, 0.6, volume_offset=(0.3, -0.2, 0.1))
+    ref = oracle.reconstruct_fdk(proj, geom, 0.8, shape, 0.6, (0.3, -0.2, 0.1), mode="f64")
+    err = cases.rel_l2(cases.to_np(vol), ref)
+    print(f"reconstruct_fdk rel L2 vs FP64 oracle {err:.3e}")
+    assert err <= 1e-5
+    vol7 = reconstruct_fdk(tp, *tg, 0.8, shape, 0.6, volume_offset=(0.3, -0.2, 0.1), angle_chunk_size=7)
+    assert cases.rel_l2(cases.to_np(vol7), cases.to_np(vol)) <= 1e-6
+
+
+def test_fdk_centred_sphere_known_answer():
+    """examples/fdk.py: analytic sphere projections reconstruct to ~1 inside."""
+    A, U, V, pix, sod, sid, radius = 90, 128, 128, 1.0, 500.0, 1000.0, 10.0
+    u = (np.arange(U) - (U - 1) / 2) * pix
+    v = (np.arange(V) - (V - 1) / 2) * pix
+    r2 = u[:, None] ** 2 + v[None, :] ** 2
+    view = 2.0 * np.sqrt(np.clip(radius ** 2 - sod ** 2 * r2 / (sid ** 2 + r2), 0, None))
+    proj = torch.tensor(np.broadcast_to(view, (A, U, V)).copy(), dtype=torch.float32, device=DEV)
+    full = lambda x: torch.full((A,), x, dtype=torch.float32, device=DEV)
+    ang = torch.arange(A, dtype=torch.float32, device=DEV) * (2 * math.pi / A)
+    u0, v0 = detector_centers(full(0.0), full(0.0), U, V, pix)
+    vol = reconstruct_fdk(proj, ang, full(sid), full(sod), u0, v0, full(0.0), full(0.0), full(0.0),
+                          pix, (64, 64, 64), 0.5)
+    D = 64
+    z, y, x = np.meshgrid(*(np.arange(D) - D / 2,) * 3, indexing="ij")
+    r = np.sqrt(x * x + y * y + z * z) * 0.5
+    v_np = cases.to_np(vol)
+    assert abs(v_np[r < 7.0].mean() - 1.0) < 0.03
+    assert v_np[r > 13.0].mean() < 0.03
+
+
+# ------------------------------------------------------------ error paths on the device
+
+def test_native_errors_raise_runtime_error():
+    import src_cbct_forwardprojector_cuda as fpn
+    vol = torch.rand(4, 4, 4, device=DEV)
+    g = [torch.zeros(3, device=DEV) for _ in range(8)]
+    with pytest.raises(RuntimeError, match="must contain A elements"):
+        fpn.forward(vol, torch.zeros(2, device=DEV), *g[1:], 1.0, 4, 4, 0.5, 1.0)
+    with pytest.raises(RuntimeError, match="must be positive"):
+        fpn.forward(vol, *g, -1.0, 4, 4, 0.5, 1.0)

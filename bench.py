@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
+    p.add_argument("--fp-tuning", default=None,
+                   help="projector kernels as use_slab,slab_thickness,lds_floats,layout (default: library's)")
     return p.parse_args()
 
 
@@ -131,6 +133,8 @@ def main():
 
     device = torch.device("cuda:0")
     torch.cuda.set_device(device)
+    if args.fp_tuning:
+        _native.set_fp_tuning(*[int(x) for x in args.fp_tuning.split(",")])
     w = WORKLOADS[args.workload]
     vol_np, geom_np, vol, geom, upstream = make_inputs(w, device)
     D, A, U, V = w["D"], w["A"], w["U"], w["V"]
@@ -188,7 +192,8 @@ def main():
         "config": {"workload": args.workload, "volume": [D, D, D], "views": A, "detector": [U, V],
                    "pixel_mm": w["pixel"], "sod_mm": w["sod"], "sid_mm": w["sid"], "voxel_mm": w["voxel"],
                    "step_mm": w["step"], "geometry_perturbed": w["perturb"], "samples": S,
-                   "t_fwd_ms": t_fwd * 1e3, "t_adj_ms": t_adj * 1e3},
+                   "t_fwd_ms": t_fwd * 1e3, "t_adj_ms": t_adj * 1e3,
+                   "fp_tuning": list(_native.get_fp_tuning())},
         "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
                      "pair_frac": (b_fwd + b_adj) / (t_fwd + t_adj) / 1e9 / HBM_PEAK_GBPS,

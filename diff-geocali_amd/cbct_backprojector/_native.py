@@ -38,6 +38,8 @@ _SIGNATURES = {
                          _F, _F, _F, _P, _P, _P, _SZ, _P], _INT),
     "cbct_fp_count_samples": ([_I64, _I64, _I64] + [_P] * 8 + [_I64, _F, _I64, _I64, _F, _F, _F, _F, _F,
                               _P, _P, _SZ, _P], _INT),
+    "cbct_fp_set_tuning": ([_INT, _INT, _INT, _INT], _INT),
+    "cbct_fp_get_tuning": ([_P], _INT),
     "cbct_bp_workspace_bytes": ([_I64, _I64, _I64, _I64], _SZ),
     "cbct_bp_forward": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
                         _P, _P, _SZ, _P], _INT),
@@ -91,6 +93,33 @@ def _workspace(nbytes, device):
 def _require_gpu(t, what):
     if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
         raise RuntimeError(f"{what} must be a GPU tensor (ROCm devices appear as 'cuda')")
+
+
+def set_fp_tuning(use_slab, slab_thickness, lds_floats, layout):
+    """Select the projector kernels for subsequent calls (same results for every setting)."""
+    _check(lib().cbct_fp_set_tuning(int(use_slab), int(slab_thickness), int(lds_floats), int(layout)))
+
+
+def get_fp_tuning():
+    out = (ctypes.c_int * 4)()
+    _check(lib().cbct_fp_get_tuning(ctypes.cast(out, ctypes.c_void_p)))
+    return tuple(out)
+
+
+class fp_tuning:
+    """Context manager that applies a projector tuning and restores the previous one."""
+
+    def __init__(self, use_slab, slab_thickness=8, lds_floats=4096, layout=0):
+        self.new = (use_slab, slab_thickness, lds_floats, layout)
+
+    def __enter__(self):
+        self.old = get_fp_tuning()
+        set_fp_tuning(*self.new)
+        return self
+
+    def __exit__(self, *exc):
+        set_fp_tuning(*self.old)
+        return False
 
 
 def fp_forward(volume, geometry, pixel_size, U, V, step_size, voxel_size, off_x, off_y, off_z):

@@ -175,7 +175,39 @@ __device__ __forceinline__ Corners corners(const Box& b, float x, float y, float
     return c;
 }
 
+// The same cell as unscaled indices: lower corner (clamped) and 0/1 steps.
+struct Cell {
+    int x0, y0, z0;
+    int sx, sy, sz;            // 1 when the +1 corner is a distinct voxel, 0 at a clamped edge
+    float fx, fy, fz;
+};
+
+__device__ __forceinline__ Cell cell_of(const Box& b, float x, float y, float z) {
+    const int xr = (int)floorf(x), yr = (int)floorf(y), zr = (int)floorf(z);
+    Cell c;
+    c.fx = x - (float)xr; c.fy = y - (float)yr; c.fz = z - (float)zr;
+    c.x0 = max(0, min(xr, b.W - 1));
+    c.y0 = max(0, min(yr, b.H - 1));
+    c.z0 = max(0, min(zr, b.D - 1));
+    c.sx = max(0, min(xr + 1, b.W - 1)) - c.x0;
+    c.sy = max(0, min(yr + 1, b.H - 1)) - c.y0;
+    c.sz = max(0, min(zr + 1, b.D - 1)) - c.z0;
+    return c;
+}
+
 struct Taps { float v000, v100, v010, v110, v001, v101, v011, v111; };
+
+// Eight taps at base offset i000 with strides (1, ry, rz) scaled by the 0/1 steps.
+__device__ __forceinline__ Taps load_taps_strided(const float* __restrict__ base, int i000,
+                                                  int dx, int dy, int dz) {
+    const float* p = base + i000;
+    Taps t;
+    t.v000 = p[0];           t.v100 = p[dx];
+    t.v010 = p[dy];          t.v110 = p[dy + dx];
+    t.v001 = p[dz];          t.v101 = p[dz + dx];
+    t.v011 = p[dz + dy];     t.v111 = p[dz + dy + dx];
+    return t;
+}
 
 __device__ __forceinline__ Taps load_taps(const float* __restrict__ vol, const Corners& c) {
     const float* p = vol + c.i000;

@@ -11,6 +11,7 @@
 // t starts at the box entry and is advanced by float addition of step_size
 // while t <= t_max.
 #include "cbct_common.h"
+#include "cbct_fp_slab.h"
 
 #include <algorithm>
 
@@ -204,6 +205,15 @@ __global__ __launch_bounds__(FP_THREADS) void fp_count_kernel(
     if (threadIdx.x == 0) atomicAdd(count, (red[0] + red[1]) + (red[2] + red[3]));
 }
 
+// Process-wide tuning of the projector pair (set by cbct_fp_set_tuning).
+struct FpTuning {
+    int use_slab = 1;     // slab-staged kernels (1) or direct kernels (0)
+    int T = 8;            // slab thickness in voxel planes
+    int cap = 4096;       // LDS floats per staged box
+    int layout = 0;       // lane layout of a tile
+};
+static FpTuning g_tuning;
+
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 static int64_t fp_tiles(int64_t U, int64_t V) { return ceil_div(U, FP_TILE) * ceil_div(V, FP_TILE); }
@@ -257,6 +267,12 @@ extern "C" int cbct_fp_forward(const float* volume, int64_t D, int64_t H, int64_
     const int tu = (int)ceil_div(U, FP_TILE), tv = (int)ceil_div(V, FP_TILE);
     const int nblocks = (int)(A * tu * tv);
     const float inv_vs = 1.0f / voxel_size;
+    if (g_tuning.use_slab) {
+        const SlabArgs args{volume, nullptr, frames, projections, nullptr, nullptr, b, (int)U, (int)V, tu, tv,
+                            nblocks, pixel_size, step_size, voxel_size, inv_vs, g_tuning.T, g_tuning.cap,
+                            g_tuning.layout};
+        return launch_fp_slab(0, !is_pow2(voxel_size), args, stream);
+    }
     if (is_pow2(voxel_size))
         fp_forward_kernel<false><<<nblocks, FP_THREADS, 0, stream>>>(
             volume, b, frames, (int)U, (int)V, tu, tv, nblocks, pixel_size, step_size, voxel_size,
@@ -299,7 +315,12 @@ extern "C" int cbct_fp_backward(const float* grad_projection, const float* volum
     const int tu = (int)ceil_div(U, FP_TILE), tv = (int)ceil_div(V, FP_TILE);
     const int nblocks = (int)(A * tu * tv);
     const float inv_vs = 1.0f / voxel_size;
-    if (compute_grad_volume)
+    if (g_tuning.use_slab) {
+        const SlabArgs args{volume, grad_projection, frames, nullptr, compute_grad_volume ? grad_volume : nullptr,
+                            partials, b, (int)U, (int)V, tu, tv, nblocks, pixel_size, step_size, voxel_size,
+                            inv_vs, g_tuning.T, g_tuning.cap, g_tuning.layout};
+        if ((st = launch_fp_slab(compute_grad_volume ? 2 : 1, false, args, stream))) return st;
+    } else if (compute_grad_volume)
         fp_backward_kernel<true><<<nblocks, FP_THREADS, 0, stream>>>(
             grad_projection, volume, b, frames, (int)U, (int)V, tu, tv, nblocks, pixel_size,
             step_size, inv_vs, grad_volume, partials);
@@ -339,5 +360,26 @@ extern "C" int cbct_fp_count_samples(int64_t D, int64_t H, int64_t W,
                                                         pixel_size, step_size,
                                                         (unsigned long long*)count);
     CBCT_LAUNCHED("sample count kernel launch");
+    return CBCT_OK;
+}
+
+extern "C" int cbct_fp_set_tuning(int use_slab, int slab_thickness, int lds_floats, int layout) {
+    CBCT_CHECK_ARG(use_slab == 0 || use_slab == 1, "use_slab must be 0 or 1");
+    CBCT_CHECK_ARG(slab_thickness >= 1 && slab_thickness <= 64, "slab_thickness must be in [1, 64]");
+    CBCT_CHECK_ARG(lds_floats >= 256 && lds_floats <= 8192, "lds_floats must be in [256, 8192]");
+    CBCT_CHECK_ARG(layout == 0 || layout == 1, "layout must be 0 or 1");
+    g_tuning.use_slab = use_slab;
+    g_tuning.T = slab_thickness;
+    g_tuning.cap = lds_floats;
+    g_tuning.layout = layout;
+    return CBCT_OK;
+}
+
+extern "C" int cbct_fp_get_tuning(int* out4) {
+    CBCT_CHECK_ARG(out4 != nullptr, "out4 must be non-null");
+    out4[0] = g_tuning.use_slab;
+    out4[1] = g_tuning.T;
+    out4[2] = g_tuning.cap;
+    out4[3] = g_tuning.layout;
     return CBCT_OK;
 }

@@ -1,0 +1,322 @@
+// Slab-staged forward projector and adjoint for gfx950.
+//
+// A workgroup owns a 16x16 tile of detector pixels of one view (one ray per
+// lane).  The tile's rays are marched together through the volume in slabs of
+// T voxel planes along the tile's principal axis (the axis the central ray
+// moves fastest along).  For each slab the workgroup
+//   1. bounds the slab's part of the tile's ray bundle: the convex hull of the
+//      four corner rays between the slab's two planes, widened by the trilinear
+//      support and a rounding margin;
+//   2. stages that box of the volume in LDS (and, for the adjoint, zeroes a
+//      second LDS box that accumulates the volume gradient);
+//   3. advances every ray through the samples whose principal voxel
+//      coordinate lies in the slab, reading the eight taps from LDS (and adding
+//      the eight weighted contributions to the LDS gradient box);
+//   4. flushes the gradient box to HBM with one atomic add per non-zero voxel.
+// The arithmetic of every sample is the same float code as the direct kernels
+// (same t schedule, same cells, same weights), so projections and geometry
+// gradients are bit-identical to them; only the order of the volume-gradient
+// atomics differs.  A sample whose taps fall outside the staged box (never in
+// a regular cone-beam geometry) and tiles whose rays do not share one principal
+// direction take the direct path from global memory, so the staging only ever
+// affects speed.
+#include "cbct_fp_slab.h"
+
+#include <algorithm>
+
+namespace cbct {
+
+__device__ __forceinline__ void pixel_dir(const ViewFrame& f, float u, float v, float pix, float r[3]) {
+    const float up = (u - f.u0) * pix, vp = (v - f.v0) * pix;
+    r[0] = fmaf(up, f.c, -f.ns) - f.ox;
+    r[1] = fmaf(up, f.s, f.nc) - f.oy;
+    r[2] = vp - f.oz;
+}
+
+__device__ __forceinline__ int box_index_div(unsigned i, unsigned magic) { return (int)__umulhi(i, magic); }
+
+// MODE 0: forward; 1: adjoint, geometry gradients only; 2: adjoint with the volume gradient.
+template <int MODE, bool DIV>
+__global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
+    extern __shared__ float lds[];
+    float* vbox = lds;
+    float* gbox = lds + P.cap;
+    const Box& b = P.b;
+
+    // ---- tile and ray of this lane
+    const int xcd = blockIdx.x & 7;
+    int tile = ((P.nblocks & 7) == 0) ? xcd * (P.nblocks >> 3) + (blockIdx.x >> 3) : (int)blockIdx.x;
+    const int tv = tile % P.tiles_v;
+    const int rest = tile / P.tiles_v;
+    const int tu = rest % P.tiles_u;
+    const int a = rest / P.tiles_u;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int du, dv;
+    if (P.layout == 0) { dv = ((wave & 1) << 3) + (lane & 7); du = ((wave >> 1) << 3) + (lane >> 3); }
+    else { dv = ((lane & 7) << 1) + (wave & 1); du = ((lane >> 3) << 1) + (wave >> 1); }
+    const int u = tu * SL_TILE + du, v = tv * SL_TILE + dv;
+    const bool inr = u < P.U && v < P.V;
+    const ViewFrame f = P.frames[a];
+
+    Ray ray;
+    float tmin = 0.f, tmax = -1.f, t = 0.f;
+    int entry = -1;
+    bool alive = false, valid = false, hit = false;
+    float go = 0.f;
+    if (inr) {
+        if (MODE != 0) go = P.grad[((int64_t)a * P.U + u) * P.V + v];
+        if ((MODE == 0 || go != 0.f) && make_ray(f, u, v, P.pix, ray)) {
+            valid = true;
+            if (intersect(b, f, ray, tmin, tmax, entry)) { hit = true; alive = true; t = tmin; }
+        }
+    }
+    const float gs = go * P.step;
+    float integral = 0.f;
+    float gox = 0.f, goy = 0.f, goz = 0.f, gdx = 0.f, gdy = 0.f, gdz = 0.f, gtmin = 0.f;
+
+    // One sample at t: taps from `src` at element offset i000 with strides (sx, sy*ry, sz*rz);
+    // the gradient contributions go to `dst` with the same addressing.
+    auto sample = [&](const float* src, float* dst, int i000, int ox1, int oy1, int oz1,
+                      float fx, float fy, float fz, bool lds_dst) {
+        const Taps tp = load_taps_strided(src, i000, ox1, oy1, oz1);
+        if (MODE == 0) {
+            integral += tri_value(tp, tri_weights(fx, fy, fz));
+        } else {
+            if (MODE == 2) {
+                const Weights w = tri_weights(fx, fy, fz);
+                float* q = dst + i000;
+                if (lds_dst) {
+                    atomicAdd(q, gs * w.w000);
+                    atomicAdd(q + ox1, gs * w.w100);
+                    atomicAdd(q + oy1, gs * w.w010);
+                    atomicAdd(q + oy1 + ox1, gs * w.w110);
+                    atomicAdd(q + oz1, gs * w.w001);
+                    atomicAdd(q + oz1 + ox1, gs * w.w101);
+                    atomicAdd(q + oz1 + oy1, gs * w.w011);
+                    atomicAdd(q + oz1 + oy1 + ox1, gs * w.w111);
+                } else {
+                    atomicAdd(q, gs * w.w000);
+                    atomicAdd(q + ox1, gs * w.w100);
+                    atomicAdd(q + oy1, gs * w.w010);
+                    atomicAdd(q + oy1 + ox1, gs * w.w110);
+                    atomicAdd(q + oz1, gs * w.w001);
+                    atomicAdd(q + oz1 + ox1, gs * w.w101);
+                    atomicAdd(q + oz1 + oy1, gs * w.w011);
+                    atomicAdd(q + oz1 + oy1 + ox1, gs * w.w111);
+                }
+            }
+            float sgx, sgy, sgz;
+            tri_gradient(tp, fx, fy, fz, sgx, sgy, sgz);
+            const float wx = sgx * P.inv_vs, wy = sgy * P.inv_vs, wz = sgz * P.inv_vs;
+            gox = fmaf(gs, wx, gox); goy = fmaf(gs, wy, goy); goz = fmaf(gs, wz, goz);
+            gdx = fmaf(gs * wx, t, gdx); gdy = fmaf(gs * wy, t, gdy); gdz = fmaf(gs * wz, t, gdz);
+            gtmin = fmaf(gs, fmaf(wz, ray.dz, fmaf(wy, ray.dy, wx * ray.dx)), gtmin);
+        }
+    };
+    auto voxel_coords = [&](float& x, float& y, float& z) {
+        const float px = fmaf(t, ray.dx, f.ox), py = fmaf(t, ray.dy, f.oy), pz = fmaf(t, ray.dz, f.oz);
+        if (MODE == 0) {
+            x = to_voxel<DIV>(px, b.offx, P.vs, P.inv_vs, b.hx);
+            y = to_voxel<DIV>(py, b.offy, P.vs, P.inv_vs, b.hy);
+            z = to_voxel<DIV>(pz, b.offz, P.vs, P.inv_vs, b.hz);
+        } else {
+            x = fmaf(px - b.offx, P.inv_vs, b.hx);
+            y = fmaf(py - b.offy, P.inv_vs, b.hy);
+            z = fmaf(pz - b.offz, P.inv_vs, b.hz);
+        }
+    };
+    const int plane = b.H * b.W;
+    auto global_sample = [&]() {
+        float x, y, z;
+        voxel_coords(x, y, z);
+        const Cell c = cell_of(b, x, y, z);
+        sample(P.vol, P.gvol, (c.z0 * b.H + c.y0) * b.W + c.x0, c.sx, c.sy * b.W, c.sz * plane,
+               c.fx, c.fy, c.fz, false);
+    };
+
+    // ---- principal axis of the tile (uniform over the workgroup)
+    const int u_lo = tu * SL_TILE, v_lo = tv * SL_TILE;
+    const int u_hi = min(u_lo + SL_TILE - 1, P.U - 1), v_hi = min(v_lo + SL_TILE - 1, P.V - 1);
+    float cr[3];
+    pixel_dir(f, 0.5f * (float)(u_lo + u_hi), 0.5f * (float)(v_lo + v_hi), P.pix, cr);
+    int p = 0;
+    if (fabsf(cr[1]) > fabsf(cr[p])) p = 1;
+    if (fabsf(cr[2]) > fabsf(cr[p])) p = 2;
+    const float dirf = cr[p] > 0.f ? 1.f : -1.f;
+    float corner[4][3];
+    pixel_dir(f, (float)u_lo, (float)v_lo, P.pix, corner[0]);
+    pixel_dir(f, (float)u_hi, (float)v_lo, P.pix, corner[1]);
+    pixel_dir(f, (float)u_lo, (float)v_hi, P.pix, corner[2]);
+    pixel_dir(f, (float)u_hi, (float)v_hi, P.pix, corner[3]);
+    bool slab_ok = true;
+    for (int j = 0; j < 4; ++j) {
+        const float n2 = corner[j][0] * corner[j][0] + corner[j][1] * corner[j][1] + corner[j][2] * corner[j][2];
+        const float cp = corner[j][p] * dirf;
+        slab_ok = slab_ok && cp > 0.f && cp * cp >= 0.0625f * n2;
+    }
+    const int Nax[3] = {b.W, b.H, b.D};
+    const float offs[3] = {b.offx, b.offy, b.offz};
+    const float halfs[3] = {b.hx, b.hy, b.hz};
+    const float src[3] = {f.ox, f.oy, f.oz};
+    float xs[3];
+    for (int i = 0; i < 3; ++i) xs[i] = (src[i] - offs[i]) * P.inv_vs + halfs[i];
+    const int o1 = p == 0 ? 1 : 0, o2 = p == 2 ? 1 : 2;
+
+    const float dpr = p == 0 ? ray.dx : (p == 1 ? ray.dy : ray.dz);
+    const bool in_slabs = alive && slab_ok && dpr * dirf > 0.f;
+    // rays that cannot follow the slab order are integrated from global memory first
+    if (alive && !in_slabs) {
+        while (alive) {
+            global_sample();
+            t += P.step;
+            alive = t <= tmax;
+        }
+    }
+
+    if (slab_ok) {
+        const int Np = Nax[p];
+        const int nslabs = (Np + 1 + P.T - 1) / P.T;   // floors -1 .. Np-1
+        for (int k = 0; k < nslabs; ++k) {
+            int s_lo, s_hi;
+            if (dirf > 0.f) { s_lo = -1 + k * P.T; s_hi = s_lo + P.T - 1; }
+            else { s_hi = Np - 1 - k * P.T; s_lo = s_hi - P.T + 1; }
+            // hull of the corner rays between the planes q = s_lo and q = s_hi + 1
+            float mn1 = FLT_MAX, mx1 = -FLT_MAX, mn2 = FLT_MAX, mx2 = -FLT_MAX;
+            bool front = true;
+            for (int j = 0; j < 4; ++j) {
+                const float rp = corner[j][p] * P.inv_vs;
+                for (int e = 0; e < 2; ++e) {
+                    const float q = (float)(e == 0 ? s_lo : s_hi + 1);
+                    const float lam = (q - xs[p]) / rp;
+                    front = front && lam > 0.f;
+                    const float c1 = fmaf(lam, corner[j][o1] * P.inv_vs, xs[o1]);
+                    const float c2 = fmaf(lam, corner[j][o2] * P.inv_vs, xs[o2]);
+                    mn1 = fminf(mn1, c1); mx1 = fmaxf(mx1, c1);
+                    mn2 = fminf(mn2, c2); mx2 = fmaxf(mx2, c2);
+                }
+            }
+            int lo[3], hi[3];
+            lo[p] = max(0, s_lo);
+            hi[p] = min(Np - 1, s_hi + 1);
+            lo[o1] = (int)fmaxf(floorf(mn1) - 1.f, 0.f);
+            hi[o1] = (int)fminf(floorf(mx1) + 2.f, (float)(Nax[o1] - 1));
+            lo[o2] = (int)fmaxf(floorf(mn2) - 1.f, 0.f);
+            hi[o2] = (int)fminf(floorf(mx2) + 2.f, (float)(Nax[o2] - 1));
+            const bool empty = !(front && lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2]);
+            const int B0 = hi[0] - lo[0] + 1, B1 = hi[1] - lo[1] + 1, B2 = hi[2] - lo[2] + 1;
+            const bool staged = !empty && B0 * B1 * B2 <= P.cap;
+            if (!staged && empty) continue;   // no sample of the tile can lie in this slab
+            const int nbox = staged ? B0 * B1 * B2 : 0;
+            const unsigned m0 = 0xffffffffu / (unsigned)B0 + 1u, m1 = 0xffffffffu / (unsigned)B1 + 1u;
+            if (staged) {
+                __syncthreads();
+                for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {
+                    const int r = box_index_div((unsigned)i, m0);
+                    const int x = lo[0] + (i - r * B0);
+                    const int z = box_index_div((unsigned)r, m1);
+                    const int y = lo[1] + (r - z * B1);
+                    vbox[i] = P.vol[((lo[2] + z) * b.H + y) * b.W + x];
+                    if (MODE == 2) gbox[i] = 0.f;
+                }
+                __syncthreads();
+            }
+            // advance the lane's ray through this slab
+            while (in_slabs && alive) {
+                float x, y, z;
+                voxel_coords(x, y, z);
+                const float cp = p == 0 ? x : (p == 1 ? y : z);
+                const int fp = (int)floorf(cp);
+                if (dirf > 0.f ? fp > s_hi : fp < s_lo) break;
+                const Cell c = cell_of(b, x, y, z);
+                const bool inside = staged && c.x0 >= lo[0] && c.x0 + c.sx <= hi[0] && c.y0 >= lo[1] &&
+                                    c.y0 + c.sy <= hi[1] && c.z0 >= lo[2] && c.z0 + c.sz <= hi[2];
+                if (inside) {
+                    const int i000 = ((c.z0 - lo[2]) * B1 + (c.y0 - lo[1])) * B0 + (c.x0 - lo[0]);
+                    const float* tapsrc = vbox;
+                    sample(tapsrc, gbox, i000, c.sx, c.sy * B0, c.sz * B0 * B1, c.fx, c.fy, c.fz, true);
+                } else {
+                    sample(P.vol, P.gvol, (c.z0 * b.H + c.y0) * b.W + c.x0, c.sx, c.sy * b.W, c.sz * plane,
+                           c.fx, c.fy, c.fz, false);
+                }
+                t += P.step;
+                alive = t <= tmax;
+            }
+            if (MODE == 2 && staged) {
+                __syncthreads();
+                for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {
+                    const float gval = gbox[i];
+                    if (gval != 0.f) {
+                        const int r = box_index_div((unsigned)i, m0);
+                        const int x = lo[0] + (i - r * B0);
+                        const int z = box_index_div((unsigned)r, m1);
+                        const int y = lo[1] + (r - z * B1);
+                        atomicAdd(P.gvol + ((lo[2] + z) * b.H + y) * b.W + x, gval);
+                    }
+                }
+            }
+        }
+    }
+    // anything left (samples the slab order did not reach)
+    while (alive) {
+        global_sample();
+        t += P.step;
+        alive = t <= tmax;
+    }
+
+    if (MODE == 0) {
+        if (inr) P.proj[((int64_t)a * P.U + u) * P.V + v] = hit ? integral * P.step : 0.f;
+        return;
+    }
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (valid) {
+        if (hit) {
+            if (entry == 0) { gox -= gtmin / ray.dx; gdx -= gtmin * tmin / ray.dx; }
+            else if (entry == 1) { goy -= gtmin / ray.dy; gdy -= gtmin * tmin / ray.dy; }
+            else if (entry == 2) { goz -= gtmin / ray.dz; gdz -= gtmin * tmin / ray.dz; }
+        }
+        const float dd = fmaf(ray.dz, gdz, fmaf(ray.dy, gdy, ray.dx * gdx));
+        const float il = 1.0f / ray.len;
+        const float grx = fmaf(-ray.dx, dd, gdx) * il;
+        const float gry = fmaf(-ray.dy, dd, gdy) * il;
+        const float grz = fmaf(-ray.dz, dd, gdz) * il;
+        const float Ox = gox - grx, Oy = goy - gry, Oz = goz - grz;
+        const float dOxb = -f.sod * f.c, dOyb = -f.sod * f.s;
+        const float dTxb = fmaf(-ray.up, f.s, -f.nc);
+        const float dTyb = fmaf(ray.up, f.c, -f.ns);
+        acc[0] = fmaf(gry, dTyb, fmaf(grx, dTxb, fmaf(Oy, dOyb, Ox * dOxb)));
+        acc[1] = fmaf(grx, f.s, -(gry * f.c));
+        acc[2] = fmaf(Oy + gry, f.c, (Ox + grx) * -f.s);
+        acc[3] = -P.pix * fmaf(gry, f.s, grx * f.c);
+        acc[4] = -P.pix * grz;
+        acc[5] = Ox; acc[6] = Oy; acc[7] = Oz;
+    }
+    __shared__ float red[SL_THREADS / 64][8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float s = wave_sum(acc[k]);
+        if (lane == 0) red[wave][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int k = threadIdx.x;
+        const float s = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+        P.partials[((int64_t)a * P.tiles_u * P.tiles_v + tu * P.tiles_v + tv) * 8 + k] = s;
+    }
+}
+
+int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream) {
+    const size_t shmem = (size_t)args.cap * sizeof(float) * (mode == 2 ? 2 : 1);
+    const dim3 grid((unsigned)args.nblocks), block(SL_THREADS);
+    if (mode == 0) {
+        if (div) fp_slab_kernel<0, true><<<grid, block, shmem, stream>>>(args);
+        else fp_slab_kernel<0, false><<<grid, block, shmem, stream>>>(args);
+    } else if (mode == 1) {
+        fp_slab_kernel<1, false><<<grid, block, shmem, stream>>>(args);
+    } else {
+        fp_slab_kernel<2, false><<<grid, block, shmem, stream>>>(args);
+    }
+    CBCT_LAUNCHED("slab projector kernel launch");
+    return CBCT_OK;
+}
+
+}  // namespace cbct

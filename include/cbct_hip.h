@@ -84,6 +84,17 @@ int cbct_fp_count_samples(int64_t D, int64_t H, int64_t W,
                           float off_x, float off_y, float off_z, uint64_t* count,
                           void* workspace, size_t workspace_bytes, void* stream);
 
+/* Process-wide tuning of cbct_fp_forward / cbct_fp_backward (no reference
+ * counterpart; set by the autotuner).  use_slab selects the slab-staged kernels
+ * (1) or the direct kernels (0); slab_thickness is the number of voxel planes
+ * per staged slab, lds_floats the LDS capacity of one staged box, layout the
+ * lane-to-pixel layout of a 16x16 ray tile (0: 8x8 blocks per wave, 1: waves
+ * interleaved with stride 2).  Every setting gives the same projections and
+ * geometry gradients bit for bit.  Not synchronised with calls in flight on
+ * other host threads. */
+int cbct_fp_set_tuning(int use_slab, int slab_thickness, int lds_floats, int layout);
+int cbct_fp_get_tuning(int* out4);
+
 /* ---- weighted backprojector: src_cbct_backprojector_cuda --------------- */
 
 /* Workspace bytes needed by cbct_bp_forward / cbct_bp_backward. */

@@ -33,7 +33,10 @@ __device__ __forceinline__ void pixel_dir(const ViewFrame& f, float u, float v, 
     r[2] = vp - f.oz;
 }
 
-__device__ __forceinline__ int box_index_div(unsigned i, unsigned magic) { return (int)__umulhi(i, magic); }
+// i / d for i < 2^16 and 1 <= d < 2^16, with magic = 0xffffffff / d + 1 (which wraps to 0 for d == 1).
+__device__ __forceinline__ int box_index_div(unsigned i, unsigned d, unsigned magic) {
+    return d == 1u ? (int)i : (int)__umulhi(i, magic);
+}
 
 // MODE 0: forward; 1: adjoint, geometry gradients only; 2: adjoint with the volume gradient.
 template <int MODE, bool DIV>
@@ -211,9 +214,9 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
             if (staged) {
                 __syncthreads();
                 for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {
-                    const int r = box_index_div((unsigned)i, m0);
+                    const int r = box_index_div((unsigned)i, (unsigned)B0, m0);
                     const int x = lo[0] + (i - r * B0);
-                    const int z = box_index_div((unsigned)r, m1);
+                    const int z = box_index_div((unsigned)r, (unsigned)B1, m1);
                     const int y = lo[1] + (r - z * B1);
                     vbox[i] = P.vol[((lo[2] + z) * b.H + y) * b.W + x];
                     if (MODE == 2) gbox[i] = 0.f;
@@ -246,9 +249,9 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
                 for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {
                     const float gval = gbox[i];
                     if (gval != 0.f) {
-                        const int r = box_index_div((unsigned)i, m0);
+                        const int r = box_index_div((unsigned)i, (unsigned)B0, m0);
                         const int x = lo[0] + (i - r * B0);
-                        const int z = box_index_div((unsigned)r, m1);
+                        const int z = box_index_div((unsigned)r, (unsigned)B1, m1);
                         const int y = lo[1] + (r - z * B1);
                         atomicAdd(P.gvol + ((lo[2] + z) * b.H + y) * b.W + x, gval);
                     }

@@ -176,36 +176,53 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
         }
     }
 
+    // Box of the tile's samples whose principal floor coordinate lies in [s_lo, s_hi]: the hull of
+    // the corner rays between the planes q = s_lo and q = s_hi + 1, widened by the +1 taps and a
+    // rounding margin, clipped to the volume.  Returns false when it is empty.
+    auto slab_box = [&](int s_lo, int s_hi, int lo[3], int hi[3]) {
+        float mn1 = FLT_MAX, mx1 = -FLT_MAX, mn2 = FLT_MAX, mx2 = -FLT_MAX;
+        bool front = true;
+        for (int j = 0; j < 4; ++j) {
+            const float rp = corner[j][p] * P.inv_vs;
+            for (int e = 0; e < 2; ++e) {
+                const float q = (float)(e == 0 ? s_lo : s_hi + 1);
+                const float lam = (q - xs[p]) / rp;
+                front = front && lam > 0.f;
+                const float c1 = fmaf(lam, corner[j][o1] * P.inv_vs, xs[o1]);
+                const float c2 = fmaf(lam, corner[j][o2] * P.inv_vs, xs[o2]);
+                mn1 = fminf(mn1, c1); mx1 = fmaxf(mx1, c1);
+                mn2 = fminf(mn2, c2); mx2 = fmaxf(mx2, c2);
+            }
+        }
+        lo[p] = max(0, s_lo);
+        hi[p] = min(Nax[p] - 1, s_hi + 1);
+        lo[o1] = (int)fminf(fmaxf(floorf(mn1) - 1.f, 0.f), (float)Nax[o1]);
+        hi[o1] = (int)fmaxf(fminf(floorf(mx1) + 2.f, (float)(Nax[o1] - 1)), -1.f);
+        lo[o2] = (int)fminf(fmaxf(floorf(mn2) - 1.f, 0.f), (float)Nax[o2]);
+        hi[o2] = (int)fmaxf(fminf(floorf(mx2) + 2.f, (float)(Nax[o2] - 1)), -1.f);
+        return front && lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2];
+    };
+
     if (slab_ok) {
         const int Np = Nax[p];
-        const int nslabs = (Np + 1 + P.T - 1) / P.T;   // floors -1 .. Np-1
-        for (int k = 0; k < nslabs; ++k) {
-            int s_lo, s_hi;
-            if (dirf > 0.f) { s_lo = -1 + k * P.T; s_hi = s_lo + P.T - 1; }
-            else { s_hi = Np - 1 - k * P.T; s_lo = s_hi - P.T + 1; }
-            // hull of the corner rays between the planes q = s_lo and q = s_hi + 1
-            float mn1 = FLT_MAX, mx1 = -FLT_MAX, mn2 = FLT_MAX, mx2 = -FLT_MAX;
-            bool front = true;
-            for (int j = 0; j < 4; ++j) {
-                const float rp = corner[j][p] * P.inv_vs;
-                for (int e = 0; e < 2; ++e) {
-                    const float q = (float)(e == 0 ? s_lo : s_hi + 1);
-                    const float lam = (q - xs[p]) / rp;
-                    front = front && lam > 0.f;
-                    const float c1 = fmaf(lam, corner[j][o1] * P.inv_vs, xs[o1]);
-                    const float c2 = fmaf(lam, corner[j][o2] * P.inv_vs, xs[o2]);
-                    mn1 = fminf(mn1, c1); mx1 = fmaxf(mx1, c1);
-                    mn2 = fminf(mn2, c2); mx2 = fmaxf(mx2, c2);
-                }
-            }
+        // Slabs cover the principal floor coordinates -1 .. Np-1 in the rays' direction; a slab is
+        // halved until its box fits the LDS capacity (uniform decision over the workgroup).
+        int next = dirf > 0.f ? -1 : Np - 1;
+        while (dirf > 0.f ? next <= Np - 1 : next >= -1) {
+            int thick = P.T, s_lo, s_hi;
             int lo[3], hi[3];
-            lo[p] = max(0, s_lo);
-            hi[p] = min(Np - 1, s_hi + 1);
-            lo[o1] = (int)fmaxf(floorf(mn1) - 1.f, 0.f);
-            hi[o1] = (int)fminf(floorf(mx1) + 2.f, (float)(Nax[o1] - 1));
-            lo[o2] = (int)fmaxf(floorf(mn2) - 1.f, 0.f);
-            hi[o2] = (int)fminf(floorf(mx2) + 2.f, (float)(Nax[o2] - 1));
-            const bool empty = !(front && lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2]);
+            bool nonempty;
+            for (;;) {
+                if (dirf > 0.f) { s_lo = next; s_hi = next + thick - 1; }
+                else { s_hi = next; s_lo = next - thick + 1; }
+                nonempty = slab_box(s_lo, s_hi, lo, hi);
+                if (!nonempty || thick == 1 ||
+                    (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) <= P.cap)
+                    break;
+                thick = (thick + 1) >> 1;
+            }
+            next += dirf > 0.f ? thick : -thick;
+            const bool empty = !nonempty;
             const int B0 = hi[0] - lo[0] + 1, B1 = hi[1] - lo[1] + 1, B2 = hi[2] - lo[2] + 1;
             const bool staged = !empty && B0 * B1 * B2 <= P.cap;
             if (!staged && empty) continue;   // no sample of the tile can lie in this slab

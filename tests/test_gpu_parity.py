@@ -152,7 +152,15 @@ def test_c1_pipeline_project_backproject_backward():
     gvol, ggeo = oracle.fp_backward(upstream, vol.astype(np.float64), geom, c["pixel"], c["step"], c["voxel"],
                                     mode="f64")
     assert cases.rel_l2(cases.to_np(tv.grad), gvol) <= 1e-5
-    assert cases.rel_l2(cases.to_np(tg[5].grad), ggeo[5]) <= 1e-4
+    # geometry gradients: the FP32 schedule itself differs from FP64 by ~1e-2 on this rough
+    # U[0,1) volume (cell assignment of samples near cell faces); the GPU matches the FP32
+    # schedule up to summation order.
+    _, g32 = oracle.fp_backward(upstream.astype(np.float32), vol, geom, c["pixel"], c["step"], c["voxel"],
+                                mode="f32")
+    e_sx32 = cases.rel_l2(cases.to_np(tg[5].grad), g32[5])
+    e_sx64 = cases.rel_l2(cases.to_np(tg[5].grad), ggeo[5])
+    print(f"c1 src_x gradient rel L2: vs FP32 schedule {e_sx32:.3e}, vs FP64 {e_sx64:.3e}")
+    assert e_sx32 <= 1e-5 and e_sx64 <= 2e-2
     bref = oracle.bp_forward(got.astype(np.float64), geom, c["pixel"], c["shape"], c["voxel"], mode="f64")
     assert cases.rel_l2(cases.to_np(bp), bref) <= 1e-5
 
@@ -174,8 +182,12 @@ def test_c1_all_nine_gradients_and_native_modules():
     gvol, ggeo = oracle.fp_backward(up.astype(np.float64), vol.astype(np.float64), geom, c["pixel"], c["step"],
                                     c["voxel"], mode="f64")
     assert cases.rel_l2(cases.to_np(grads[0]), gvol) <= 1e-5
+    _, g32 = oracle.fp_backward(up, vol, geom, c["pixel"], c["step"], c["voxel"], mode="f32")
     for k in range(8):
-        assert cases.rel_l2(cases.to_np(grads[k + 1]), ggeo[k]) <= 1e-4, k
+        e32 = cases.rel_l2(cases.to_np(grads[k + 1]), g32[k])
+        e64 = cases.rel_l2(cases.to_np(grads[k + 1]), ggeo[k])
+        print(f"c1 geometry gradient {k}: rel L2 vs FP32 schedule {e32:.3e}, vs FP64 {e64:.3e}")
+        assert e32 <= 1e-5 and e64 <= 2e-2, k
         np.testing.assert_array_equal(cases.to_np(g_nov[k + 1]), cases.to_np(grads[k + 1]))
     vb = bpn.forward(proj, *tg, c["pixel"], *c["shape"], c["voxel"])
     gv = rng.standard_normal(vb.shape).astype(np.float32)
@@ -308,7 +320,11 @@ def test_projector_kernel_variants_agree(tuning):
         _, gg0 = _native.fp_backward(up, vol, geom, 0.9, 0.25, 0.5, False, 0.3, -0.2, 0.1)
     torch.cuda.synchronize()
     assert torch.equal(p, ref_p)
-    assert torch.equal(gg, ref_gg) and torch.equal(gg0, ref_gg)
+    if tuning[3] == 0:       # same lane layout as the direct kernel: same summation order
+        assert torch.equal(gg, ref_gg) and torch.equal(gg0, ref_gg)
+    else:                    # another lane order changes only the FP32 summation order
+        assert cases.rel_l2(cases.to_np(gg), cases.to_np(ref_gg)) <= 1e-6
+        assert torch.equal(gg0, gg)
     assert cases.rel_l2(cases.to_np(gv), cases.to_np(ref_gv)) <= 1e-6
     ref32 = oracle.fp_forward(cases.to_np(vol), [cases.to_np(g) for g in geom], 0.9, U, V, 0.25, 0.5,
                               (0.3, -0.2, 0.1), mode="f32")

@@ -75,6 +75,54 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
     }
     const float gs = go * P.step;
     float integral = 0.f;
+    // Fixed-point scale of the LDS gradient box (MODE 2): a power of two 2^e chosen so that the
+    // largest total a voxel can receive from this tile within one slab stays below 2^30.  That
+    // total is at most max|gs| * n_rays * n_smp, where n_smp bounds the samples of one ray inside
+    // a voxel's 2x2x2 support (floor(2 sqrt3 vs / step) + 1) and n_rays bounds the tile's rays
+    // through that support (pixels within its projected radius, or all 256).
+    float gsq = 0.f, inv_scale = 1.f;
+    __shared__ float red_max[SL_THREADS / 64];
+    unsigned* gboxu = reinterpret_cast<unsigned*>(gbox);
+    if (MODE == 2) {
+        float m = fabsf(gs);
+        for (int s = 32; s >= 1; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
+        if ((threadIdx.x & 63) == 0) red_max[threadIdx.x >> 6] = m;
+        __syncthreads();
+        const float amax = fmaxf(fmaxf(red_max[0], red_max[1]), fmaxf(red_max[2], red_max[3]));
+        // distance from the source to the volume box and the widest ray angle of the tile
+        const float ddx = fmaxf(fmaxf(b.lox - f.ox, f.ox - b.hix), 0.f);
+        const float ddy = fmaxf(fmaxf(b.loy - f.oy, f.oy - b.hiy), 0.f);
+        const float ddz = fmaxf(fmaxf(b.loz - f.oz, f.oz - b.hiz), 0.f);
+        const float dmin = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
+        float cmin = 1.f;
+        {
+            const int ul = tu * SL_TILE, vl = tv * SL_TILE;
+            const float us[2] = {(float)ul, (float)min(ul + SL_TILE - 1, P.U - 1)};
+            const float vv[2] = {(float)vl, (float)min(vl + SL_TILE - 1, P.V - 1)};
+            for (int i = 0; i < 4; ++i) {
+                float r[3];
+                pixel_dir(f, us[i & 1], vv[i >> 1], P.pix, r);
+                const float rn = (-f.s * r[0] + f.c * r[1]);     // component along the detector normal
+                const float len = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                cmin = fminf(cmin, fabsf(rn) / len);
+            }
+        }
+        float n_rays = (float)SL_THREADS;
+        if (dmin > 0.f && cmin > 0.2f) {
+            const float rpx = 1.05f * 1.7320508f * P.vs * f.sid / (dmin * P.pix * cmin * cmin);
+            n_rays = fminf(n_rays, (2.f * rpx + 2.f) * (2.f * rpx + 2.f));
+        }
+        const float n_smp = floorf(2.f * 1.7320508f * P.vs / P.step) + 1.f;
+        const float bound = amax * n_rays * n_smp;
+        int e = 0;
+        if (bound > 0.f) {
+            int k;
+            frexpf(1073741824.f / bound, &k);     // 2^30 / bound = m * 2^k, m in [0.5, 1)
+            e = max(-100, min(100, k - 1));
+        }
+        gsq = gs * ldexpf(1.f, e);
+        inv_scale = ldexpf(1.f, -e);
+    }
     float gox = 0.f, goy = 0.f, goz = 0.f, gdx = 0.f, gdy = 0.f, gdz = 0.f, gtmin = 0.f;
 
     // One sample at t: taps from `src` at element offset i000 with strides (sx, sy*ry, sz*rz);
@@ -89,14 +137,16 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
                 const Weights w = tri_weights(fx, fy, fz);
                 float* q = dst + i000;
                 if (lds_dst) {
-                    atomicAdd(q, gs * w.w000);
-                    atomicAdd(q + ox1, gs * w.w100);
-                    atomicAdd(q + oy1, gs * w.w010);
-                    atomicAdd(q + oy1 + ox1, gs * w.w110);
-                    atomicAdd(q + oz1, gs * w.w001);
-                    atomicAdd(q + oz1 + ox1, gs * w.w101);
-                    atomicAdd(q + oz1 + oy1, gs * w.w011);
-                    atomicAdd(q + oz1 + oy1 + ox1, gs * w.w111);
+                    // fixed-point accumulation: integer LDS atomics run at full rate on gfx950
+                    unsigned* qi = gboxu + i000;
+                    atomicAdd(qi, (unsigned)__float2int_rn(gsq * w.w000));
+                    atomicAdd(qi + ox1, (unsigned)__float2int_rn(gsq * w.w100));
+                    atomicAdd(qi + oy1, (unsigned)__float2int_rn(gsq * w.w010));
+                    atomicAdd(qi + oy1 + ox1, (unsigned)__float2int_rn(gsq * w.w110));
+                    atomicAdd(qi + oz1, (unsigned)__float2int_rn(gsq * w.w001));
+                    atomicAdd(qi + oz1 + ox1, (unsigned)__float2int_rn(gsq * w.w101));
+                    atomicAdd(qi + oz1 + oy1, (unsigned)__float2int_rn(gsq * w.w011));
+                    atomicAdd(qi + oz1 + oy1 + ox1, (unsigned)__float2int_rn(gsq * w.w111));
                 } else {
                     atomicAdd(q, gs * w.w000);
                     atomicAdd(q + ox1, gs * w.w100);
@@ -236,7 +286,7 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
                     const int z = box_index_div((unsigned)r, (unsigned)B1, m1);
                     const int y = lo[1] + (r - z * B1);
                     vbox[i] = P.vol[((lo[2] + z) * b.H + y) * b.W + x];
-                    if (MODE == 2) gbox[i] = 0.f;
+                    if (MODE == 2) gboxu[i] = 0u;
                 }
                 __syncthreads();
             }
@@ -264,8 +314,9 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
             if (MODE == 2 && staged) {
                 __syncthreads();
                 for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {
-                    const float gval = gbox[i];
-                    if (gval != 0.f) {
+                    const int ival = (int)gboxu[i];
+                    if (ival != 0) {
+                        const float gval = (float)ival * inv_scale;
                         const int r = box_index_div((unsigned)i, (unsigned)B0, m0);
                         const int x = lo[0] + (i - r * B0);
                         const int z = box_index_div((unsigned)r, (unsigned)B1, m1);

@@ -210,7 +210,7 @@ struct FpTuning {
     int use_slab = 1;     // slab-staged kernels (1) or direct kernels (0)
     int T = 8;            // slab thickness in voxel planes
     int cap = 4096;       // LDS floats per staged box
-    int layout = 0;       // lane layout of a tile
+    int layout = 1;       // lane layout of a tile
 };
 static FpTuning g_tuning;
 

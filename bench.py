@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
+    p.add_argument("--pmc-probe", action="store_true",
+                   help="run a 1 GiB reference copy, one forward and one adjoint (for rocprofv3 --pmc runs) and exit")
     p.add_argument("--fp-tuning", default=None,
                    help="projector kernels as use_slab,slab_thickness,lds_floats,layout (default: library's)")
     return p.parse_args()
@@ -139,6 +141,19 @@ def main():
     vol_np, geom_np, vol, geom, upstream = make_inputs(w, device)
     D, A, U, V = w["D"], w["A"], w["U"], w["V"]
     scal = (w["pixel"], U, V, w["step"], w["voxel"], 0.0, 0.0, 0.0)
+    if args.pmc_probe:
+        # reference copy of known size (1 GiB read + 1 GiB written) to scale FETCH_SIZE / WRITE_SIZE
+        a = torch.ones(1 << 28, device=device)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        _native.fp_forward(vol, geom, *scal)
+        _native.fp_backward(upstream, vol, geom, w["pixel"], w["step"], w["voxel"], True, 0.0, 0.0, 0.0)
+        torch.cuda.synchronize()
+        print("pmc probe done", flush=True)
+        return
     S = _native.fp_count_samples((D, D, D), geom, w["pixel"], U, V, w["step"], w["voxel"])
 
     fdk = w.get("fdk", False)

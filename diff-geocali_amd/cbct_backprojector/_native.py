@@ -49,6 +49,8 @@ _SIGNATURES = {
     "cbct_fdk_filter": ([_P, _I64, _I64, _I64, _F, _P, _P, _F, _P, _P, _SZ, _P], _INT),
     "cbct_fdk_reconstruct": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
                              _F, _I64, _P, _P, _SZ, _P], _INT),
+    "cbct_calib_workspace_bytes": ([_P, _INT], _SZ),
+    "cbct_calib_run": ([_P, _INT, _INT, _P, _F, _F, _F, _INT, _P, _SZ, _P], _INT),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)

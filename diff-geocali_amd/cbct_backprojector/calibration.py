@@ -1,0 +1,189 @@
+"""Gradient-based per-view geometry calibration on the HIP library.
+
+The reference stops at the differentiable operators and leaves the calibration
+loop to user code (examples/autograd.py:62-63; the full pipeline is excluded,
+README.md:8).  This module is that loop, MI355X-native: each problem is a known
+volume and a measured projection stack, and its eight per-view geometry arrays
+are fitted by Adam to the mean squared reprojection residual
+
+    L(g) = mean over the A*U*V rays of (project(volume, g) - measured)^2 .
+
+``calibrate`` runs many problems of uneven size together: one step of all of
+them is four launches (frames, one fused march that gives each ray's
+projection, residual and geometry gradient, a fixed-order per-view reduction,
+the Adam update), and all steps can be captured in one hipGraph
+(``cbct_calib_run``).
+
+``reprojection_error_px`` is the accuracy measure used to judge a fit: the RMS
+over views and nine fixed points (the volume's corners and centre) of the
+detector-position difference, in pixels, between two geometries, with the
+point-to-detector map of the backprojector
+(csrc/src_cbct_backprojector_cuda.cu:48-72).
+"""
+
+import ctypes
+import math
+
+import torch
+
+from . import _native
+
+GEOMETRY_NAMES = ("angles", "sid", "sod", "det_u0", "det_v0", "src_x", "src_y", "src_z")
+
+# Default Adam learning rates per parameter group, in the parameters' units
+# (rad, mm, mm, px, px, mm, mm, mm): about a tenth of the typical miscalibration.
+DEFAULT_LR = (3e-4, 1.0, 0.6, 0.15, 0.15, 0.05, 0.05, 0.05)
+
+
+class _CalibProblem(ctypes.Structure):
+    _fields_ = [
+        ("volume", ctypes.c_void_p), ("D", ctypes.c_int64), ("H", ctypes.c_int64), ("W", ctypes.c_int64),
+        ("voxel_size", ctypes.c_float), ("off_x", ctypes.c_float), ("off_y", ctypes.c_float),
+        ("off_z", ctypes.c_float),
+        ("measured", ctypes.c_void_p), ("A", ctypes.c_int64), ("U", ctypes.c_int64), ("V", ctypes.c_int64),
+        ("pixel_size", ctypes.c_float), ("step_size", ctypes.c_float),
+        ("geometry", ctypes.c_void_p), ("adam_m", ctypes.c_void_p), ("adam_v", ctypes.c_void_p),
+        ("loss_history", ctypes.c_void_p), ("grad_out", ctypes.c_void_p),
+        ("slab_thickness", ctypes.c_int), ("lds_floats", ctypes.c_int), ("layout", ctypes.c_int),
+    ]
+
+
+def stack_geometry(geometry):
+    """[8, A] float32 tensor from a dict or a sequence of eight [A] tensors."""
+    if isinstance(geometry, dict):
+        geometry = [geometry[name] for name in GEOMETRY_NAMES]
+    if isinstance(geometry, torch.Tensor):
+        if geometry.ndim != 2 or geometry.shape[0] != 8:
+            raise ValueError("geometry tensor must have shape [8, A]")
+        return geometry.to(torch.float32).contiguous()
+    if len(geometry) != 8:
+        raise ValueError("geometry needs eight per-view tensors")
+    return torch.stack([torch.as_tensor(g).reshape(-1).to(torch.float32) for g in geometry]).contiguous()
+
+
+class CalibrationProblem:
+    """One scan to calibrate: a known volume, its measured projections and a starting geometry."""
+
+    def __init__(self, volume, measured, geometry, pixel_size, step_size, voxel_size,
+                 volume_offset=(0.0, 0.0, 0.0), tuning=None):
+        if volume.dtype != torch.float32 or volume.ndim != 3 or volume.device.type != "cuda":
+            raise ValueError("volume must be a [D,H,W] float32 GPU tensor")
+        if measured.dtype != torch.float32 or measured.ndim != 3 or measured.device != volume.device:
+            raise ValueError("measured must be an [A,U,V] float32 tensor on the volume's device")
+        self.volume = volume.contiguous()
+        self.measured = measured.contiguous()
+        self.geometry = stack_geometry(geometry).to(volume.device).clone()
+        if self.geometry.shape[1] != measured.shape[0]:
+            raise ValueError("geometry must have one value per view of measured")
+        self.pixel_size = float(pixel_size)
+        self.step_size = float(step_size)
+        self.voxel_size = float(voxel_size)
+        self.volume_offset = tuple(float(x) for x in volume_offset)
+        self.tuning = tuning              # (slab_thickness, lds_floats, layout) or None
+        A = measured.shape[0]
+        self.adam_m = torch.zeros((8, A), dtype=torch.float32, device=volume.device)
+        self.adam_v = torch.zeros_like(self.adam_m)
+        self.loss_history = None
+        self.grad = None
+
+    def _struct(self, nsteps, want_grad):
+        self.loss_history = torch.zeros(nsteps + 1, dtype=torch.float32, device=self.volume.device)
+        self.grad = torch.zeros_like(self.geometry) if want_grad else None
+        D, H, W = self.volume.shape
+        A, U, V = self.measured.shape
+        t = self.tuning or (0, 0, 0)
+        return _CalibProblem(
+            self.volume.data_ptr(), D, H, W, self.voxel_size, *self.volume_offset,
+            self.measured.data_ptr(), A, U, V, self.pixel_size, self.step_size,
+            self.geometry.data_ptr(), self.adam_m.data_ptr(), self.adam_v.data_ptr(),
+            self.loss_history.data_ptr(), self.grad.data_ptr() if want_grad else None,
+            int(t[0]), int(t[1]), int(t[2]))
+
+
+def calibrate(problems, steps=20, lr=DEFAULT_LR, betas=(0.9, 0.999), eps=1e-8, use_graph=True,
+              want_grad=False):
+    """Run ``steps`` Adam steps on every problem together; problems are updated in place
+    (``problem.geometry``, ``problem.adam_m/v``, ``problem.loss_history`` of length steps+1,
+    and with ``want_grad`` the loss gradient at the final geometry in ``problem.grad``)."""
+    if isinstance(problems, CalibrationProblem):
+        problems = [problems]
+    problems = list(problems)
+    if not problems:
+        raise ValueError("no problems")
+    device = problems[0].volume.device
+    if any(p.volume.device != device for p in problems):
+        raise ValueError("all problems must be on one device")
+    lib = _native.lib()
+    arr = (_CalibProblem * len(problems))(*[p._struct(steps, want_grad) for p in problems])
+    lr_arr = (ctypes.c_float * 8)(*[float(x) for x in lr])
+    with torch.cuda.device(device):
+        nbytes = lib.cbct_calib_workspace_bytes(ctypes.addressof(arr), len(problems))
+        ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+        status = lib.cbct_calib_run(ctypes.addressof(arr), len(problems), int(steps), ctypes.addressof(lr_arr),
+                                    float(betas[0]), float(betas[1]), float(eps), 1 if use_graph else 0,
+                                    ctypes.c_void_p(ws.data_ptr()), nbytes, stream)
+    if status != 0:
+        raise RuntimeError(lib.cbct_last_error().decode())
+    return problems
+
+
+def fixed_points(volume_shape, voxel_size, volume_offset=(0.0, 0.0, 0.0)):
+    """The volume's eight corner voxel centres and its centre, world coordinates [9, 3] (x, y, z)."""
+    D, H, W = volume_shape
+    pts = []
+    for z in (0, D - 1):
+        for y in (0, H - 1):
+            for x in (0, W - 1):
+                pts.append(((x - W * 0.5) * voxel_size + volume_offset[0],
+                            (y - H * 0.5) * voxel_size + volume_offset[1],
+                            (z - D * 0.5) * voxel_size + volume_offset[2]))
+    pts.append((volume_offset[0] - 0.5 * voxel_size, volume_offset[1] - 0.5 * voxel_size,
+                volume_offset[2] - 0.5 * voxel_size))
+    return torch.tensor(pts, dtype=torch.float64)
+
+
+def detector_positions(geometry, points, pixel_size):
+    """(u_idx, v_idx) [A, n] of world points under each view's geometry, FP64
+    (the map of csrc/src_cbct_backprojector_cuda.cu:48-72)."""
+    g = stack_geometry(geometry).to(torch.float64)
+    P = points.to(device=g.device, dtype=torch.float64)
+    ang, sid, sod, u0, v0, sx, sy, sz = [x[:, None] for x in g]
+    c, s = torch.cos(ang), torch.sin(ang)
+    px = P[None, :, 0] * c + P[None, :, 1] * s
+    py = -P[None, :, 0] * s + P[None, :, 1] * c
+    pz = P[None, :, 2]
+    dsu = sx * c + sy * s
+    dsn = -sx * s + sy * c
+    t = (sid + dsn) / (sod + dsn - py)
+    u = (dsu + t * (px - dsu)) / pixel_size + u0
+    v = (sz + t * (pz - sz)) / pixel_size + v0
+    return u, v
+
+
+def reprojection_error_px(geometry, reference, volume_shape, voxel_size, pixel_size,
+                          volume_offset=(0.0, 0.0, 0.0)):
+    """RMS over views and the nine fixed points of the detector-position difference (pixels)."""
+    pts = fixed_points(volume_shape, voxel_size, volume_offset)
+    u1, v1 = detector_positions(geometry, pts, pixel_size)
+    u2, v2 = detector_positions(reference, pts, pixel_size)
+    return math.sqrt(float(((u1 - u2) ** 2 + (v1 - v2) ** 2).mean()))
+
+
+def synthetic_problem(spec, device, perturb=True, tuning=None):
+    """A calibration problem on a seeded ellipsoid phantom: the measured projections are the
+    projector's output at a perturbed geometry (returned as ``true_geometry``), and the fit
+    starts from the nominal circular scan.  ``spec``: D, A, U, V, pixel, voxel, sod, sid, seed."""
+    from . import phantom
+    D, A, U, V = spec["D"], spec["A"], spec["U"], spec["V"]
+    vs, pix = spec["voxel"], spec["pixel"]
+    params = phantom.ellipsoid_parameters(spec["seed"], 12, half_width=D * vs / 2)
+    volume = torch.tensor(phantom.rasterize_ellipsoids(params, (D, D, D), vs), device=device)
+    nominal = phantom.circular_geometry(A, spec["sid"], spec["sod"], U, V, pix)
+    true = phantom.perturb_geometry(nominal, spec["seed"] + 1) if perturb else nominal
+    true_list = [true[k].to(device) for k in GEOMETRY_NAMES]
+    step = spec.get("step", vs / 2)
+    measured = _native.fp_forward(volume, true_list, pix, U, V, step, vs, 0.0, 0.0, 0.0)
+    problem = CalibrationProblem(volume, measured, nominal, pix, step, vs, tuning=tuning)
+    problem.true_geometry = stack_geometry(true_list)
+    return problem

@@ -99,3 +99,20 @@ def perturb_geometry(geometry, seed, angle_deg=0.2, center_px=1.5, src_mm=0.5, d
 
 def geometry_list(geometry):
     return [geometry[k] for k in GEOMETRY_NAMES]
+
+
+def calibration_specs(count, seed, side=(96, 320), views=(90, 720), detector=(128, 768), voxel_size=0.5,
+                      sod=600.0, sid=1100.0):
+    """Sizes of ``count`` uneven calibration problems: cubic volumes of log-uniform side,
+    a uniform-integer number of views, square detectors of uniform-integer side.  The pixel
+    size is chosen so that the magnified volume diagonal fills 95% of the detector."""
+    rng = np.random.default_rng(int(seed))
+    specs = []
+    for i in range(count):
+        D = int(round(math.exp(rng.uniform(math.log(side[0]), math.log(side[1])))))
+        A = int(rng.integers(views[0], views[1] + 1))
+        U = int(rng.integers(detector[0], detector[1] + 1))
+        pixel = (sid / sod) * math.sqrt(2.0) * D * voxel_size / (0.95 * U)
+        specs.append({"D": D, "A": A, "U": U, "V": U, "pixel": pixel, "voxel": voxel_size, "sod": sod,
+                      "sid": sid, "seed": int(seed) * 1000 + i})
+    return specs

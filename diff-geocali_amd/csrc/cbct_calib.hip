@@ -1,0 +1,269 @@
+// Batched gradient-based geometry calibration for gfx950.
+//
+// Each problem is a known volume and a measured projection stack; its eight
+// per-view geometry arrays are fitted by Adam to the mean squared reprojection
+// residual  L = mean over rays of (project(volume, geometry) - measured)^2.
+// One optimisation step of all problems is
+//   1. frames      per-view frames of every problem from the current geometry;
+//   2. march       the persistent mode-3 slab kernel: every ray of every problem
+//                  is marched once, giving its projection, its squared residual and
+//                  the geometry gradient of the loss (no separate forward pass);
+//   3. reduce      per-view sums of the per-tile partials, in a fixed order;
+//   4. adam        the Adam update of every geometry value.
+// The steps can be captured once in a hipGraph and launched as one graph.
+// Problems of uneven size share every launch; persistent workgroups pull tiles
+// from one counter, which balances the uneven work.
+#include "cbct_common.h"
+#include "cbct_fp_slab.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace cbct {
+
+__device__ __forceinline__ int find_problem(const CalibDev* __restrict__ probs, int nprob, int idx, int which) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int off = which == 0 ? probs[mid].view_offset : probs[mid].param_offset;
+        if (off <= idx) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void calib_zero_kernel(unsigned* __restrict__ counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0u;
+}
+
+__global__ void calib_frames_kernel(const CalibDev* __restrict__ probs, int nprob, int total_views) {
+    const int gv = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gv >= total_views) return;
+    const int p = find_problem(probs, nprob, gv, 0);
+    const CalibDev& d = probs[p];
+    const int a = gv - d.view_offset, A = d.A;
+    const float* g = d.geom;
+    const GeomPtrs gp{g, g + A, g + 2 * A, g + 3 * A, g + 4 * A, g + 5 * A, g + 6 * A, g + 7 * A};
+    const_cast<ViewFrame*>(d.s.frames)[a] = make_frame(gp, a);
+}
+
+// Per view: fixed-order sum of the tile partials (8 geometry gradients + loss).
+__global__ __launch_bounds__(256) void calib_reduce_kernel(const CalibDev* __restrict__ probs, int nprob,
+                                                           int step) {
+    const int gv = blockIdx.x;
+    const int p = find_problem(probs, nprob, gv, 0);
+    const CalibDev& d = probs[p];
+    const int a = gv - d.view_offset;
+    const int tiles = d.s.tiles_u * d.s.tiles_v;
+    float acc[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int t = threadIdx.x; t < tiles; t += 256) {
+        const float* q = d.s.partials + ((int64_t)a * tiles + t) * 9;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) acc[k] += q[k];
+    }
+    __shared__ float sm[4][9];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const float s = wave_sum(acc[k]);
+        if (lane == 0) sm[wave][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        const int k = threadIdx.x;
+        const float s = (sm[0][k] + sm[1][k]) + (sm[2][k] + sm[3][k]);
+        if (k < 8) d.ggeo[k * d.A + a] = s;
+        else {
+            d.vloss[a] = s;
+            atomicAdd(d.loss_hist + step, s);
+        }
+    }
+}
+
+struct AdamArgs {
+    float lr[8];
+    float beta1, beta2, eps, bc1, bc2;   // bias corrections 1 - beta^t
+};
+
+__global__ void calib_adam_kernel(const CalibDev* __restrict__ probs, int nprob, int total_params, AdamArgs h) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= total_params) return;
+    const int p = find_problem(probs, nprob, gi, 1);
+    const CalibDev& d = probs[p];
+    const int li = gi - d.param_offset;
+    const int k = li / d.A;
+    const float g = d.ggeo[li];
+    const float m = h.beta1 * d.m[li] + (1.0f - h.beta1) * g;
+    const float v = h.beta2 * d.v[li] + (1.0f - h.beta2) * g * g;
+    d.m[li] = m;
+    d.v[li] = v;
+    const float mhat = m / h.bc1, vhat = v / h.bc2;
+    d.geom[li] -= h.lr[k] * mhat / (sqrtf(vhat) + h.eps);
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct CalibLayout {
+    size_t descs, counter, total;
+    std::vector<size_t> frames, partials, ggeo, vloss;
+};
+
+static int64_t tiles_of(int64_t U, int64_t V) { return ceil_div(U, SL_TILE) * ceil_div(V, SL_TILE); }
+
+static CalibLayout calib_layout(const cbct_calib_problem* pr, int n) {
+    CalibLayout L;
+    size_t off = 0;
+    L.descs = off; off += align256(sizeof(CalibDev) * (size_t)n);
+    L.counter = off; off += 256;
+    for (int i = 0; i < n; ++i) {
+        const int64_t A = pr[i].A;
+        L.frames.push_back(off); off += align256((size_t)A * sizeof(ViewFrame));
+        L.partials.push_back(off); off += align256((size_t)A * tiles_of(pr[i].U, pr[i].V) * 9 * sizeof(float));
+        L.ggeo.push_back(off); off += align256((size_t)A * 8 * sizeof(float));
+        L.vloss.push_back(off); off += align256((size_t)A * sizeof(float));
+    }
+    L.total = off;
+    return L;
+}
+
+int launch_fill(float* p, int64_t n, float value, hipStream_t stream);
+
+}  // namespace cbct
+
+using namespace cbct;
+
+extern "C" size_t cbct_calib_workspace_bytes(const cbct_calib_problem* problems, int nproblems) {
+    if (!problems || nproblems < 1) return 0;
+    return calib_layout(problems, nproblems).total;
+}
+
+extern "C" int cbct_calib_run(const cbct_calib_problem* pr, int nproblems, int nsteps, const float* lr8,
+                              float beta1, float beta2, float eps, int use_graph,
+                              void* workspace, size_t workspace_bytes, void* stream_) {
+    CBCT_CHECK_ARG(pr != nullptr && nproblems >= 1, "at least one problem is required");
+    CBCT_CHECK_ARG(nsteps >= 0 && nsteps <= 100000, "nsteps must be in [0, 100000]");
+    CBCT_CHECK_ARG(lr8 != nullptr, "lr must point to 8 learning rates");
+    CBCT_CHECK_ARG(beta1 >= 0.f && beta1 < 1.f && beta2 >= 0.f && beta2 < 1.f && eps > 0.f,
+                   "Adam parameters out of range");
+    hipStream_t stream = (hipStream_t)stream_;
+    const CalibLayout L = calib_layout(pr, nproblems);
+    CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= L.total, "workspace too small for cbct_calib_run");
+    char* ws = (char*)workspace;
+    int dflt[4] = {1, 8, 4096, 1};
+    cbct_fp_get_tuning(dflt);
+    std::vector<CalibDev> host(nproblems);
+    int total_tiles = 0, total_views = 0, total_params = 0, max_cap = 256;
+    for (int i = 0; i < nproblems; ++i) {
+        const cbct_calib_problem& q = pr[i];
+        CBCT_CHECK_ARG(q.volume && q.measured && q.geometry && q.adam_m && q.adam_v && q.loss_history,
+                       "problem pointers must be non-null");
+        CBCT_CHECK_ARG(q.D > 0 && q.H > 0 && q.W > 0 && q.D * q.H * q.W <= INT32_MAX,
+                       "volume dimensions must be positive and within the 2^31-1 element limit");
+        CBCT_CHECK_ARG(q.A > 0 && q.U > 0 && q.V > 0 && q.A * tiles_of(q.U, q.V) <= INT32_MAX,
+                       "projection dimensions must be positive");
+        int st;
+        if ((st = check_positive_finite(q.pixel_size, "pixel_size"))) return st;
+        if ((st = check_positive_finite(q.step_size, "step_size"))) return st;
+        if ((st = check_positive_finite(q.voxel_size, "voxel_size"))) return st;
+        CalibDev& d = host[i];
+        SlabArgs& s = d.s;
+        s.vol = q.volume;
+        s.grad = q.measured;
+        s.frames = (const ViewFrame*)(ws + L.frames[i]);
+        s.proj = nullptr;
+        s.gvol = nullptr;
+        s.partials = (float*)(ws + L.partials[i]);
+        s.b = make_box_host(q.D, q.H, q.W, q.voxel_size, q.off_x, q.off_y, q.off_z);
+        s.U = (int)q.U; s.V = (int)q.V;
+        s.tiles_u = (int)ceil_div(q.U, SL_TILE); s.tiles_v = (int)ceil_div(q.V, SL_TILE);
+        s.nblocks = (int)(q.A * s.tiles_u * s.tiles_v);
+        s.pix = q.pixel_size; s.step = q.step_size; s.vs = q.voxel_size; s.inv_vs = 1.0f / q.voxel_size;
+        s.T = q.slab_thickness > 0 ? q.slab_thickness : dflt[1];
+        s.cap = q.lds_floats > 0 ? q.lds_floats : dflt[2];
+        s.layout = q.layout >= 0 && q.layout <= 1 && q.slab_thickness > 0 ? q.layout : dflt[3];
+        CBCT_CHECK_ARG(s.T >= 1 && s.T <= 64 && s.cap >= 256 && s.cap <= 8192, "slab tuning out of range");
+        s.resid_scale = (float)(2.0 / ((double)q.A * (double)q.U * (double)q.V));
+        d.geom = q.geometry; d.m = q.adam_m; d.v = q.adam_v;
+        d.ggeo = q.grad_out ? q.grad_out : (float*)(ws + L.ggeo[i]);
+        d.vloss = (float*)(ws + L.vloss[i]);
+        d.loss_hist = q.loss_history;
+        d.A = (int)q.A;
+        d.div = is_pow2(q.voxel_size) ? 0 : 1;
+        d.view_offset = total_views;
+        d.tile_offset = total_tiles;
+        d.param_offset = total_params;
+        total_views += (int)q.A;
+        total_tiles += s.nblocks;
+        total_params += 8 * (int)q.A;
+        max_cap = std::max(max_cap, s.cap);
+    }
+    CalibDev* ddev = (CalibDev*)(ws + L.descs);
+    unsigned* counter = (unsigned*)(ws + L.counter);
+    // descriptors: a one-time upload, ordered on the stream before every step
+    CBCT_HIP(hipMemcpyAsync(ddev, host.data(), sizeof(CalibDev) * nproblems, hipMemcpyHostToDevice, stream),
+             "descriptor upload");
+    CBCT_HIP(hipStreamSynchronize(stream), "descriptor upload");
+    for (int i = 0; i < nproblems; ++i) {
+        int st;
+        if ((st = launch_fill(pr[i].loss_history, nsteps + 1, 0.0f, stream))) return st;
+    }
+    int device = 0, cus = 0, per_cu = 0;
+    CBCT_HIP(hipGetDevice(&device), "hipGetDevice");
+    CBCT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
+    const int grid = std::max(1, std::min(total_tiles, cus * 8));
+    (void)per_cu;
+    AdamArgs h;
+    for (int k = 0; k < 8; ++k) h.lr[k] = lr8[k];
+    h.beta1 = beta1; h.beta2 = beta2; h.eps = eps;
+
+    auto enqueue = [&](int step, bool update, hipStream_t st_) -> int {
+        calib_zero_kernel<<<1, 64, 0, st_>>>(counter);
+        CBCT_LAUNCHED("calibration counter reset");
+        calib_frames_kernel<<<(unsigned)ceil_div(total_views, 128), 128, 0, st_>>>(ddev, nproblems, total_views);
+        CBCT_LAUNCHED("calibration frames kernel launch");
+        int st;
+        if ((st = launch_fp_slab_batched(ddev, nproblems, total_tiles, max_cap, counter, grid, st_))) return st;
+        calib_reduce_kernel<<<(unsigned)total_views, 256, 0, st_>>>(ddev, nproblems, step);
+        CBCT_LAUNCHED("calibration reduce kernel launch");
+        if (update) {
+            h.bc1 = (float)(1.0 - std::pow((double)beta1, step + 1));
+            h.bc2 = (float)(1.0 - std::pow((double)beta2, step + 1));
+            calib_adam_kernel<<<(unsigned)ceil_div(total_params, 256), 256, 0, st_>>>(ddev, nproblems,
+                                                                                      total_params, h);
+            CBCT_LAUNCHED("calibration Adam kernel launch");
+        }
+        return CBCT_OK;
+    };
+
+    if (use_graph) {
+        // Capture on a private stream (the caller's may be the null stream, which cannot be
+        // captured), then launch the graph on the caller's stream.
+        hipStream_t cs = nullptr;
+        CBCT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) { (void)hipStreamDestroy(cs); return hip_status(e, "graph capture begin"); }
+        int st = CBCT_OK;
+        for (int s = 0; s < nsteps && st == CBCT_OK; ++s) st = enqueue(s, true, cs);
+        if (st == CBCT_OK) st = enqueue(nsteps, false, cs);
+        e = hipStreamEndCapture(cs, &graph);
+        (void)hipStreamDestroy(cs);
+        if (st != CBCT_OK) { if (graph) (void)hipGraphDestroy(graph); return st; }
+        if (e != hipSuccess) return hip_status(e, "graph capture end");
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) { (void)hipGraphDestroy(graph); return hip_status(e, "graph instantiate"); }
+        e = hipGraphLaunch(exec, stream);
+        hipError_t e2 = hipStreamSynchronize(stream);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return hip_status(e, "graph launch");
+        if (e2 != hipSuccess) return hip_status(e2, "graph execution");
+        return CBCT_OK;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        int st;
+        if ((st = enqueue(s, true, stream))) return st;
+    }
+    return enqueue(nsteps, false, stream);
+}

@@ -21,9 +21,28 @@ struct SlabArgs {
     int T;        // slab thickness in voxel planes
     int cap;      // LDS floats per box
     int layout;   // 0: each wave owns an 8x8 pixel block; 1: waves interleave with stride 2
+    float resid_scale;   // mode 3: dL/dproj = resid_scale * (proj - measured)
+};
+
+// One calibration problem of a batched launch (device resident).
+struct CalibDev {
+    SlabArgs s;          // mode-3 projector problem; s.grad = measured projections
+    float* geom;         // [8][A] geometry, updated in place
+    float* m;            // [8][A] Adam first moments
+    float* v;            // [8][A] Adam second moments
+    float* ggeo;         // [8][A] gradient of the loss
+    float* vloss;        // [A] loss per view
+    float* loss_hist;    // [nsteps] loss before each update
+    int A, view_offset, tile_offset, param_offset;
+    int div;             // voxel size not a power of two: the forward's division map
 };
 
 // mode 0: forward; 1: adjoint (geometry only); 2: adjoint with the volume gradient.
 int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream);
+
+// Mode 3 (fused forward + squared residual + geometry adjoint) over every tile of every problem,
+// persistent workgroups fed by the work counter (which must be zero at launch).
+int launch_fp_slab_batched(const CalibDev* probs, int nprob, int total_tiles, int max_cap,
+                           unsigned* counter, int grid, hipStream_t stream);
 
 }  // namespace cbct

@@ -38,17 +38,19 @@ __device__ __forceinline__ int box_index_div(unsigned i, unsigned d, unsigned ma
     return d == 1u ? (int)i : (int)__umulhi(i, magic);
 }
 
-// MODE 0: forward; 1: adjoint, geometry gradients only; 2: adjoint with the volume gradient.
+// One 16x16 tile of rays of one problem.
+// MODE 0: forward; 1: adjoint, geometry gradients only; 2: adjoint with the volume gradient;
+// 3: calibration step -- forward, squared residual against the measured projections (P.grad) and
+//    the geometry gradient of the loss, in one march (the geometry sums are linear in the upstream
+//    gradient, so they are accumulated with unit weight and scaled by the residual at the end).
 template <int MODE, bool DIV>
-__global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
+__device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     extern __shared__ float lds[];
     float* vbox = lds;
     float* gbox = lds + P.cap;
     const Box& b = P.b;
 
     // ---- tile and ray of this lane
-    const int xcd = blockIdx.x & 7;
-    int tile = ((P.nblocks & 7) == 0) ? xcd * (P.nblocks >> 3) + (blockIdx.x >> 3) : (int)blockIdx.x;
     const int tv = tile % P.tiles_v;
     const int rest = tile / P.tiles_v;
     const int tu = rest % P.tiles_u;
@@ -68,12 +70,12 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
     float go = 0.f;
     if (inr) {
         if (MODE != 0) go = P.grad[((int64_t)a * P.U + u) * P.V + v];
-        if ((MODE == 0 || go != 0.f) && make_ray(f, u, v, P.pix, ray)) {
+        if ((MODE == 0 || MODE == 3 || go != 0.f) && make_ray(f, u, v, P.pix, ray)) {
             valid = true;
             if (intersect(b, f, ray, tmin, tmax, entry)) { hit = true; alive = true; t = tmin; }
         }
     }
-    const float gs = go * P.step;
+    const float gs = MODE == 3 ? 1.0f : go * P.step;
     float integral = 0.f;
     // Fixed-point scale of the LDS gradient box (MODE 2): a power of two 2^e chosen so that the
     // largest total a voxel can receive from this tile within one slab stays below 2^30.  That
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
 
     auto voxel_coords = [&](float& x, float& y, float& z) {
         const float px = fmaf(t, ray.dx, f.ox), py = fmaf(t, ray.dy, f.oy), pz = fmaf(t, ray.dz, f.oz);
-        if (MODE == 0) {
+        if (MODE == 0 || MODE == 3) {      // the forward's voxel map (division unless vs is a power of two)
             x = to_voxel<DIV>(px, b.offx, P.vs, P.inv_vs, b.hx);
             y = to_voxel<DIV>(py, b.offy, P.vs, P.inv_vs, b.hy);
             z = to_voxel<DIV>(pz, b.offz, P.vs, P.inv_vs, b.hz);
@@ -139,9 +141,8 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
     };
     // Everything a sample does once its eight taps are known.
     auto use_taps = [&](const Taps& tp, float fx, float fy, float fz) {
-        if (MODE == 0) {
-            integral += tri_value(tp, tri_weights(fx, fy, fz));
-        } else {
+        if (MODE == 0 || MODE == 3) integral += tri_value(tp, tri_weights(fx, fy, fz));
+        if (MODE != 0) {
             float sgx, sgy, sgz;
             tri_gradient(tp, fx, fy, fz, sgx, sgy, sgz);
             const float wx = sgx * P.inv_vs, wy = sgy * P.inv_vs, wz = sgz * P.inv_vs;
@@ -346,6 +347,14 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
         if (inr) P.proj[((int64_t)a * P.U + u) * P.V + v] = hit ? integral * P.step : 0.f;
         return;
     }
+    float loss = 0.f;
+    if (MODE == 3 && inr) {
+        // residual of this ray; scale the unit-weight geometry sums by d(loss)/d(proj) * step
+        const float r = (hit ? integral * P.step : 0.f) - go;
+        loss = 0.5f * P.resid_scale * r * r;
+        const float g = P.resid_scale * r * P.step;
+        gox *= g; goy *= g; goz *= g; gdx *= g; gdy *= g; gdz *= g; gtmin *= g;
+    }
     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (valid) {
         if (hit) {
@@ -369,18 +378,62 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
         acc[4] = -P.pix * grz;
         acc[5] = Ox; acc[6] = Oy; acc[7] = Oz;
     }
-    __shared__ float red[SL_THREADS / 64][8];
+    constexpr int NP = MODE == 3 ? 9 : 8;
+    __shared__ float red[SL_THREADS / 64][9];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const float s = wave_sum(acc[k]);
         if (lane == 0) red[wave][k] = s;
     }
+    if (MODE == 3) {
+        const float s = wave_sum(loss);
+        if (lane == 0) red[wave][8] = s;
+    }
     __syncthreads();
-    if (threadIdx.x < 8) {
+    if (threadIdx.x < NP) {
         const int k = threadIdx.x;
         const float s = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
-        P.partials[((int64_t)a * P.tiles_u * P.tiles_v + tu * P.tiles_v + tv) * 8 + k] = s;
+        P.partials[((int64_t)a * P.tiles_u * P.tiles_v + tu * P.tiles_v + tv) * NP + k] = s;
     }
+}
+
+template <int MODE, bool DIV>
+__global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
+    // Workgroups are dispatched round-robin over the 8 XCDs; permute so that consecutive tiles of a
+    // view share an XCD (and its L2).
+    const int xcd = blockIdx.x & 7;
+    const int tile = ((P.nblocks & 7) == 0) ? xcd * (P.nblocks >> 3) + (blockIdx.x >> 3) : (int)blockIdx.x;
+    slab_tile<MODE, DIV>(P, tile);
+}
+
+// Persistent calibration kernel: workgroups pull tiles of all problems from one counter.
+__global__ __launch_bounds__(SL_THREADS) void fp_slab_batched_kernel(const CalibDev* __restrict__ probs,
+                                                                     int nprob, int total_tiles,
+                                                                     unsigned* __restrict__ counter) {
+    __shared__ int sh_tile;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) sh_tile = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int tile = sh_tile;
+        if (tile >= total_tiles) break;
+        int lo = 0, hi = nprob - 1;            // last problem whose tile_offset <= tile
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (probs[mid].tile_offset <= tile) lo = mid; else hi = mid - 1;
+        }
+        const SlabArgs P = probs[lo].s;
+        if (probs[lo].div) slab_tile<3, true>(P, tile - probs[lo].tile_offset);
+        else slab_tile<3, false>(P, tile - probs[lo].tile_offset);
+    }
+}
+
+int launch_fp_slab_batched(const CalibDev* probs, int nprob, int total_tiles, int max_cap,
+                           unsigned* counter, int grid, hipStream_t stream) {
+    fp_slab_batched_kernel<<<grid, SL_THREADS, (size_t)max_cap * sizeof(float), stream>>>(
+        probs, nprob, total_tiles, counter);
+    CBCT_LAUNCHED("batched calibration kernel launch");
+    return CBCT_OK;
 }
 
 int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream) {

@@ -150,6 +150,42 @@ int cbct_fdk_reconstruct(const float* projections, int64_t A, int64_t U, int64_t
                          float alpha, int64_t angle_chunk_size,
                          float* volume, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- geometry calibration ------------------------------------------------
+ * No reference counterpart: the reference leaves the calibration loop to user
+ * code (examples/autograd.py:62-63, README.md:8).  Fits the eight per-view
+ * geometry arrays of each problem by Adam to the mean squared reprojection
+ * residual  L = mean over the A*U*V rays of (projection - measured)^2,  where
+ * the projection is cbct_fp_forward's.  One step marches every ray once (the
+ * forward value, the residual and the geometry gradient in one pass). */
+typedef struct {
+    const float* volume;       /* [D,H,W] known volume */
+    int64_t D, H, W;
+    float voxel_size;
+    float off_x, off_y, off_z;
+    const float* measured;     /* [A,U,V] measured projections (line integrals) */
+    int64_t A, U, V;
+    float pixel_size, step_size;
+    float* geometry;           /* [8][A] geometry, updated in place */
+    float* adam_m;             /* [8][A] Adam moments; zero them before a fresh fit */
+    float* adam_v;
+    float* loss_history;       /* [nsteps + 1]: L before each update and after the last */
+    float* grad_out;           /* optional [8][A]: dL/dgeometry at the final geometry */
+    int slab_thickness;        /* kernel tuning of this problem; 0 = library tuning */
+    int lds_floats;
+    int layout;
+} cbct_calib_problem;
+
+size_t cbct_calib_workspace_bytes(const cbct_calib_problem* problems, int nproblems);
+
+/* Runs nsteps Adam steps on all problems together (uneven sizes share each
+ * launch; persistent workgroups balance them).  lr8: learning rate per
+ * parameter group in the geometry order.  use_graph != 0 captures all steps in
+ * one hipGraph and launches it once.  The descriptor upload and, with a graph,
+ * the end of the run synchronise the stream. */
+int cbct_calib_run(const cbct_calib_problem* problems, int nproblems, int nsteps, const float* lr8,
+                   float beta1, float beta2, float eps, int use_graph,
+                   void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

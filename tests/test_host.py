@@ -1,0 +1,66 @@
+"""Host-side logic that needs no GPU: the uneven calibration problem sizes, the
+reprojection-error measure, and the calibration entry point's argument checks."""
+
+import ctypes
+import math
+
+import torch
+
+from cbct_backprojector import _native, calibration, phantom
+
+
+def test_calibration_specs_are_uneven_and_in_range():
+    specs = phantom.calibration_specs(64, seed=4)
+    sides = [s["D"] for s in specs]
+    assert min(sides) >= 96 and max(sides) <= 320 and len(set(sides)) > 20
+    assert all(90 <= s["A"] <= 720 and 128 <= s["U"] == s["V"] <= 768 for s in specs)
+    # log-uniform side: about half of the problems below the geometric mean of the range
+    below = sum(s["D"] < math.sqrt(96 * 320) for s in specs)
+    assert 20 <= below <= 44
+    assert phantom.calibration_specs(64, seed=4) == specs
+
+
+def test_reprojection_error_zero_for_same_geometry_and_exact_for_detector_shift():
+    geom = phantom.circular_geometry(36, 1100.0, 600.0, 128, 128, 1.0)
+    shape = (64, 64, 64)
+    assert calibration.reprojection_error_px(geom, geom, shape, 0.5, 1.0) == 0.0
+    shifted = dict(geom)
+    shifted["det_u0"] = geom["det_u0"] + 0.3
+    shifted["det_v0"] = geom["det_v0"] - 0.4
+    # a detector-centre shift moves every point by exactly (0.3, -0.4) px
+    err = calibration.reprojection_error_px(shifted, geom, shape, 0.5, 1.0)
+    assert abs(err - 0.5) < 1e-6
+
+
+def test_detector_positions_follow_the_backprojector_map():
+    """The centre of rotation projects to the detector centre; a point on the source side
+    at (0, y) for angle 0 is magnified by sid / (sod - y)."""
+    geom = phantom.circular_geometry(4, 1000.0, 500.0, 100, 80, 0.5)
+    pts = torch.tensor([[0.0, 0.0, 0.0], [10.0, 100.0, 4.0]], dtype=torch.float64)
+    u, v = calibration.detector_positions(geom, pts, 0.5)
+    assert abs(u[0, 0].item() - 49.5) < 1e-9 and abs(v[0, 0].item() - 39.5) < 1e-9
+    mag = 1000.0 / (500.0 - 100.0)
+    assert abs(u[0, 1].item() - (49.5 + 10.0 * mag / 0.5)) < 1e-9
+    assert abs(v[0, 1].item() - (39.5 + 4.0 * mag / 0.5)) < 1e-9
+
+
+def test_calib_entry_point_rejects_bad_arguments_without_a_gpu():
+    lib = _native.lib()
+    lr = (ctypes.c_float * 8)(*([0.1] * 8))
+    assert lib.cbct_calib_workspace_bytes(None, 0) == 0
+    st = lib.cbct_calib_run(None, 0, 5, ctypes.addressof(lr), 0.9, 0.999, 1e-8, 0, None, 0, None)
+    assert st != 0 and b"at least one problem" in lib.cbct_last_error()
+    prob = calibration._CalibProblem()
+    prob.D = prob.H = prob.W = 8
+    prob.A = prob.U = prob.V = 4
+    st = lib.cbct_calib_run(ctypes.addressof(prob), 1, -1, ctypes.addressof(lr), 0.9, 0.999, 1e-8, 0, None, 0,
+                            None)
+    assert st != 0 and b"nsteps" in lib.cbct_last_error()
+    st = lib.cbct_calib_run(ctypes.addressof(prob), 1, 3, ctypes.addressof(lr), 1.5, 0.999, 1e-8, 0, None, 0,
+                            None)
+    assert st != 0 and b"Adam" in lib.cbct_last_error()
+    nbytes = lib.cbct_calib_workspace_bytes(ctypes.addressof(prob), 1)
+    assert nbytes > 0
+    st = lib.cbct_calib_run(ctypes.addressof(prob), 1, 3, ctypes.addressof(lr), 0.9, 0.999, 1e-8, 0, None, 0,
+                            None)
+    assert st != 0 and b"workspace" in lib.cbct_last_error()

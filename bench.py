@@ -37,6 +37,9 @@ WORKLOADS = {
     "fdk-roundtrip-256c-360v": dict(D=256, A=360, U=512, V=384, pixel=0.8, sod=500.0, sid=1000.0,
                                     voxel=0.5, step=0.25, perturb=False, fdk=True),
 }
+CALIB_WORKLOAD = "batched-calibration-64p"
+WORKLOADS[CALIB_WORKLOAD] = dict(problems=64, adam_steps=20, side=(96, 320), views=(90, 720), detector=(128, 768),
+                                 voxel=0.5, sod=600.0, sid=1100.0, seed=4)
 DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 PHANTOM_SEED = 20261203
 PERTURB_SEED = 7
@@ -52,6 +55,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     p.add_argument("--pmc-probe", action="store_true",
                    help="run a 1 GiB reference copy, one forward and one adjoint (for rocprofv3 --pmc runs) and exit")
+    p.add_argument("--adam-steps", type=int, default=None,
+                   help="Adam steps per problem of the batched-calibration workload (default 20)")
     p.add_argument("--fp-tuning", default=None,
                    help="projector kernels as use_slab,slab_thickness,lds_floats,layout (default: library's)")
     return p.parse_args()
@@ -122,6 +127,87 @@ def cpu_baseline(w, vol_np, geom_np, budget_s):
                       f"FP32 reference-schedule C oracle, {t_total:.1f} s"}
 
 
+def calibration_bench(args, torch, device):
+    """Batched calibration: 64 uneven problems (cubic phantoms of log-uniform side 96-320,
+    90-720 views, square detectors 128-768, geometry perturbed as in the 256^3 x 720 workload),
+    each fitted by 20 Adam steps in one hipGraph of persistent launches; brick shapes from the
+    autotuner.  One step of this bench = one whole batched calibration (20 Adam steps plus the
+    final loss evaluation = 21 fused marches per problem).  GUPS counts D^3*A voxel-view updates
+    per march."""
+    from cbct_backprojector import _native, autotune, calibration, phantom
+    w = WORKLOADS[CALIB_WORKLOAD]
+    specs = phantom.calibration_specs(w["problems"], w["seed"], w["side"], w["views"], w["detector"], w["voxel"],
+                                      w["sod"], w["sid"])
+    t0 = time.perf_counter()
+    probs = [calibration.synthetic_problem(s, device) for s in specs]
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for p in probs:
+        autotune.tune_calibration(p, views=4, reps=1)
+    tune_s = time.perf_counter() - t0
+    print(f"# {len(probs)} problems set up in {setup_s:.1f} s, tuned in {tune_s:.1f} s", file=sys.stderr, flush=True)
+    starts = [p.geometry.clone() for p in probs]
+    samples = 0
+    for p, s in zip(probs, specs):
+        samples += _native.fp_count_samples((s["D"],) * 3, list(p.geometry), s["pixel"], s["U"], s["V"],
+                                            p.step_size, p.voxel_size)
+    nsteps = w["adam_steps"] if args.adam_steps is None else args.adam_steps
+
+    def reset():
+        for p, g in zip(probs, starts):
+            p.geometry.copy_(g)
+            p.adam_m.zero_()
+            p.adam_v.zero_()
+
+    for _ in range(args.warmup):
+        reset()
+        t0 = time.perf_counter()
+        calibration.calibrate(probs, steps=nsteps, use_graph=True)
+        print(f"# warm-up calibration {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    times = []
+    t_wall = 0.0
+    for _ in range(args.steps):
+        reset()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        calibration.calibrate(probs, steps=nsteps, use_graph=True)
+        e1.record()
+        torch.cuda.synchronize()
+        t_wall += time.perf_counter() - t0
+        times.append(e0.elapsed_time(e1) / 1e3)
+    t = sum(times) / len(times)
+    marches = nsteps + 1
+    updates = sum(s["D"] ** 3 * s["A"] for s in specs) * marches
+    rays = sum(s["A"] * s["U"] * s["V"] for s in specs)
+    # fused march taps: 32 B per trilinear sample (the gradient reuses them) + 4 B measured per ray
+    bytes_all = (32.0 * samples + 4.0 * rays) * marches
+    achieved = bytes_all / t / 1e9
+    err0 = [calibration.reprojection_error_px(g, p.true_geometry, tuple(p.volume.shape), p.voxel_size,
+                                              p.pixel_size) for g, p in zip(starts, probs)]
+    err1 = [calibration.reprojection_error_px(p.geometry, p.true_geometry, tuple(p.volume.shape), p.voxel_size,
+                                              p.pixel_size) for p in probs]
+    loss_ratio = [p.loss_history[-1].item() / p.loss_history[0].item() for p in probs]
+    med = lambda x: sorted(x)[len(x) // 2]
+    out = {
+        "metric": METRIC, "value": updates / t / 1e9, "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": CALIB_WORKLOAD, "problems": len(probs), "adam_steps": nsteps,
+                   "s_per_batched_calibration": t, "samples_per_march": samples, "rays": rays,
+                   "voxel_view_updates_per_march": updates // marches,
+                   "setup_s": setup_s, "autotune_s": tune_s,
+                   "reprojection_px_median_before": med(err0), "reprojection_px_median_after": med(err1),
+                   "loss_ratio_median": med(loss_ratio),
+                   "tunings": sorted({str(p.tuning) for p in probs})},
+        "roofline": {"bound": "hbm", "kernel": "fused calibration march (whole graph timed)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -135,6 +221,8 @@ def main():
 
     device = torch.device("cuda:0")
     torch.cuda.set_device(device)
+    if args.workload == CALIB_WORKLOAD:
+        return calibration_bench(args, torch, device)
     if args.fp_tuning:
         _native.set_fp_tuning(*[int(x) for x in args.fp_tuning.split(",")])
     w = WORKLOADS[args.workload]

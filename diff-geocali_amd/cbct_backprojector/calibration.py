@@ -178,7 +178,7 @@ def synthetic_problem(spec, device, perturb=True, tuning=None):
     D, A, U, V = spec["D"], spec["A"], spec["U"], spec["V"]
     vs, pix = spec["voxel"], spec["pixel"]
     params = phantom.ellipsoid_parameters(spec["seed"], 12, half_width=D * vs / 2)
-    volume = torch.tensor(phantom.rasterize_ellipsoids(params, (D, D, D), vs), device=device)
+    volume = phantom.rasterize_ellipsoids_on(params, (D, D, D), vs, device)
     nominal = phantom.circular_geometry(A, spec["sid"], spec["sod"], U, V, pix)
     true = phantom.perturb_geometry(nominal, spec["seed"] + 1) if perturb else nominal
     true_list = [true[k].to(device) for k in GEOMETRY_NAMES]

@@ -61,6 +61,31 @@ def rasterize_ellipsoids(params, shape, voxel_size, offset=(0.0, 0.0, 0.0)):
     return vol
 
 
+def rasterize_ellipsoids_on(params, shape, voxel_size, device, offset=(0.0, 0.0, 0.0)):
+    """The phantom of ``rasterize_ellipsoids`` evaluated with torch on ``device`` (FP64
+    inside test), as a float32 [D,H,W] tensor; for large synthetic batches."""
+    D, H, W = shape
+    vol = torch.zeros((D, H, W), dtype=torch.float32, device=device)
+    f64 = dict(dtype=torch.float64, device=device)
+    xs = (torch.arange(W, **f64) - W * 0.5) * voxel_size + offset[0]
+    ys = (torch.arange(H, **f64) - H * 0.5) * voxel_size + offset[1]
+    zs = (torch.arange(D, **f64) - D * 0.5) * voxel_size + offset[2]
+    for e in np.asarray(params, np.float64).reshape(-1, 11):
+        c, ax, R, rho = e[0:3], e[3:6], quaternion_matrix(e[6:10]), e[10]
+        reach = np.abs(R) @ ax
+        sel = [torch.nonzero((coords - c[k]).abs() <= reach[k]).flatten()
+               for coords, k in ((xs, 0), (ys, 1), (zs, 2))]
+        if any(s.numel() == 0 for s in sel):
+            continue
+        ix, iy, iz = sel
+        Z, Y, X = torch.meshgrid(zs[iz] - c[2], ys[iy] - c[1], xs[ix] - c[0], indexing="ij")
+        P = torch.stack([X, Y, Z], dim=-1) @ torch.tensor(R, **f64)
+        inside = ((P / torch.tensor(ax, **f64)) ** 2).sum(-1) <= 1.0
+        z0, y0, x0 = int(iz[0]), int(iy[0]), int(ix[0])
+        vol[z0:z0 + len(iz), y0:y0 + len(iy), x0:x0 + len(ix)] += (rho * inside).to(torch.float32)
+    return vol
+
+
 def circular_geometry(A, sid, sod, U, V, pixel_size, device="cpu", start=0.0):
     """Eight float32 [A] tensors of a uniform full-circle scan with a centred detector."""
     angles = torch.arange(A, dtype=torch.float64) * (2.0 * math.pi / A) + start

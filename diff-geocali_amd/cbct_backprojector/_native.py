@@ -50,7 +50,26 @@ _SIGNATURES = {
     "cbct_fdk_reconstruct": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
                              _F, _I64, _P, _P, _SZ, _P], _INT),
     "cbct_calib_workspace_bytes": ([_P, _INT], _SZ),
-    "cbct_calib_run": ([_P, _INT, _INT, _P, _F, _F, _F, _INT, _P, _SZ, _P], _INT),
+    "cbct_calib_run": ([_P, _INT, _INT, _INT, _P, _F, _F, _F, _F, _INT, _P, _SZ, _P], _INT),
+    # learned warm start (include/cbct_nn.h)
+    "cbct_nn_gemm_workspace_bytes": ([_I64, _I64, _INT], _SZ),
+    "cbct_nn_gemm": ([_INT, _INT, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _INT, _P, _I64, _P, _I64, _INT,
+                      _P, _SZ, _P], _INT),
+    "cbct_nn_im2col": ([_P, _I64, _I64, _I64, _I64, _INT, _I64, _P, _P], _INT),
+    "cbct_nn_col2im": ([_P, _I64, _I64, _I64, _I64, _INT, _I64, _P, _P, _P], _INT),
+    "cbct_nn_relu_backward": ([_P, _P, _I64, _P, _P, _P], _INT),
+    "cbct_nn_spatial_mean": ([_P, _I64, _I64, _I64, _P, _P], _INT),
+    "cbct_nn_spatial_mean_backward": ([_P, _I64, _I64, _I64, _P, _P, _P], _INT),
+    "cbct_nn_view_concat": ([_P, _I64, _I64, _I64, _P, _P], _INT),
+    "cbct_nn_view_concat_backward": ([_P, _I64, _I64, _I64, _P, _P], _INT),
+    "cbct_nn_colsum": ([_P, _I64, _I64, _P, _P], _INT),
+    "cbct_nn_mse": ([_P, _P, _I64, _P, _P, _P], _INT),
+    "cbct_nn_adam": ([_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _F, _F, _F, _P], _INT),
+    "cbct_nn_to_bf16": ([_P, _I64, _P, _P], _INT),
+    "cbct_nn_prepare_input": ([_P, _P, _I64, _I64, _I64, _INT, _F, _F, _P, _P], _INT),
+    "cbct_nn_jacobian_mse": ([_P, _P, _P, _I64, _I64, _I64, _I64, _P, _P, _P], _INT),
+    "cbct_nn_sqdiff_workspace_bytes": ([], _SZ),
+    "cbct_nn_sqdiff_mean": ([_P, _P, _I64, _P, _P, _SZ, _P], _INT),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)

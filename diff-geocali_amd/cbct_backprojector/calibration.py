@@ -101,10 +101,12 @@ class CalibrationProblem:
 
 
 def calibrate(problems, steps=20, lr=DEFAULT_LR, betas=(0.9, 0.999), eps=1e-8, use_graph=True,
-              want_grad=False):
+              want_grad=False, lr_decay=1.0, first_step=0):
     """Run ``steps`` Adam steps on every problem together; problems are updated in place
     (``problem.geometry``, ``problem.adam_m/v``, ``problem.loss_history`` of length steps+1,
-    and with ``want_grad`` the loss gradient at the final geometry in ``problem.grad``)."""
+    and with ``want_grad`` the loss gradient at the final geometry in ``problem.grad``).
+    Step t (counted from ``first_step``; continue a fit by passing the steps already taken)
+    uses the learning rates ``lr * lr_decay**t``."""
     if isinstance(problems, CalibrationProblem):
         problems = [problems]
     problems = list(problems)
@@ -120,9 +122,10 @@ def calibrate(problems, steps=20, lr=DEFAULT_LR, betas=(0.9, 0.999), eps=1e-8, u
         nbytes = lib.cbct_calib_workspace_bytes(ctypes.addressof(arr), len(problems))
         ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
         stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
-        status = lib.cbct_calib_run(ctypes.addressof(arr), len(problems), int(steps), ctypes.addressof(lr_arr),
-                                    float(betas[0]), float(betas[1]), float(eps), 1 if use_graph else 0,
-                                    ctypes.c_void_p(ws.data_ptr()), nbytes, stream)
+        status = lib.cbct_calib_run(ctypes.addressof(arr), len(problems), int(steps), int(first_step),
+                                    ctypes.addressof(lr_arr), float(lr_decay), float(betas[0]), float(betas[1]),
+                                    float(eps), 1 if use_graph else 0, ctypes.c_void_p(ws.data_ptr()), nbytes,
+                                    stream)
     if status != 0:
         raise RuntimeError(lib.cbct_last_error().decode())
     return problems

@@ -137,11 +137,13 @@ extern "C" size_t cbct_calib_workspace_bytes(const cbct_calib_problem* problems,
     return calib_layout(problems, nproblems).total;
 }
 
-extern "C" int cbct_calib_run(const cbct_calib_problem* pr, int nproblems, int nsteps, const float* lr8,
-                              float beta1, float beta2, float eps, int use_graph,
+extern "C" int cbct_calib_run(const cbct_calib_problem* pr, int nproblems, int nsteps, int first_step,
+                              const float* lr8, float lr_decay, float beta1, float beta2, float eps, int use_graph,
                               void* workspace, size_t workspace_bytes, void* stream_) {
     CBCT_CHECK_ARG(pr != nullptr && nproblems >= 1, "at least one problem is required");
     CBCT_CHECK_ARG(nsteps >= 0 && nsteps <= 100000, "nsteps must be in [0, 100000]");
+    CBCT_CHECK_ARG(first_step >= 0 && first_step <= 100000000, "first_step must be in [0, 1e8]");
+    CBCT_CHECK_ARG(lr_decay > 0.f && lr_decay <= 1.f, "lr_decay must be in (0, 1]");
     CBCT_CHECK_ARG(lr8 != nullptr, "lr must point to 8 learning rates");
     CBCT_CHECK_ARG(beta1 >= 0.f && beta1 < 1.f && beta2 >= 0.f && beta2 < 1.f && eps > 0.f,
                    "Adam parameters out of range");
@@ -226,8 +228,11 @@ extern "C" int cbct_calib_run(const cbct_calib_problem* pr, int nproblems, int n
         calib_reduce_kernel<<<(unsigned)total_views, 256, 0, st_>>>(ddev, nproblems, step);
         CBCT_LAUNCHED("calibration reduce kernel launch");
         if (update) {
-            h.bc1 = (float)(1.0 - std::pow((double)beta1, step + 1));
-            h.bc2 = (float)(1.0 - std::pow((double)beta2, step + 1));
+            const int t = first_step + step;
+            const double decay = std::pow((double)lr_decay, t);
+            for (int k = 0; k < 8; ++k) h.lr[k] = (float)(lr8[k] * decay);
+            h.bc1 = (float)(1.0 - std::pow((double)beta1, t + 1));
+            h.bc2 = (float)(1.0 - std::pow((double)beta2, t + 1));
             calib_adam_kernel<<<(unsigned)ceil_div(total_params, 256), 256, 0, st_>>>(ddev, nproblems,
                                                                                       total_params, h);
             CBCT_LAUNCHED("calibration Adam kernel launch");

@@ -179,11 +179,14 @@ size_t cbct_calib_workspace_bytes(const cbct_calib_problem* problems, int nprobl
 
 /* Runs nsteps Adam steps on all problems together (uneven sizes share each
  * launch; persistent workgroups balance them).  lr8: learning rate per
- * parameter group in the geometry order.  use_graph != 0 captures all steps in
- * one hipGraph and launches it once.  The descriptor upload and, with a graph,
- * the end of the run synchronise the stream. */
-int cbct_calib_run(const cbct_calib_problem* problems, int nproblems, int nsteps, const float* lr8,
-                   float beta1, float beta2, float eps, int use_graph,
+ * parameter group in the geometry order; step t (counted from first_step, so a
+ * fit can be continued by a later call with the same Adam moments) uses
+ * lr8 * lr_decay^t and the bias corrections 1 - beta^(t+1).  loss_history[i]
+ * is the loss before the i-th update of this call.  use_graph != 0 captures all
+ * steps in one hipGraph and launches it once.  The descriptor upload and, with
+ * a graph, the end of the run synchronise the stream. */
+int cbct_calib_run(const cbct_calib_problem* problems, int nproblems, int nsteps, int first_step,
+                   const float* lr8, float lr_decay, float beta1, float beta2, float eps, int use_graph,
                    void* workspace, size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus

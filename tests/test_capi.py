@@ -13,11 +13,12 @@ import cases  # noqa: F401  (sets sys.path)
 from cbct_backprojector import _native
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "cbct_hip.h")
+HEADERS = [os.path.join(ROOT, "include", name) for name in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if name.endswith(".h")]
 
 
 def declared_symbols():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     return sorted(set(re.findall(r"\b(cbct_[a-z0-9_]+)\s*\(", text)))
 
 

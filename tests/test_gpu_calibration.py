@@ -122,3 +122,15 @@ def test_calibration_reduces_reprojection_error():
     print(f"40 steps: loss {hist[0]:.3e} -> {hist[-1]:.3e}; reprojection error {e0:.3f} px -> {e1:.3f} px")
     assert hist[-1] < 0.2 * hist[0]
     assert e1 < 0.5 * e0
+
+
+def test_continued_fit_equals_one_long_fit():
+    """calibrate(a steps) then calibrate(b steps, first_step=a) with the same Adam moments
+    equals calibrate(a + b steps), learning-rate decay included."""
+    p1 = _problem(SMALL)
+    calibration.calibrate([p1], steps=5, use_graph=True, lr_decay=0.9)
+    p2 = _problem(SMALL)
+    calibration.calibrate([p2], steps=2, use_graph=False, lr_decay=0.9)
+    calibration.calibrate([p2], steps=3, use_graph=True, lr_decay=0.9, first_step=2)
+    assert torch.equal(p1.geometry, p2.geometry)
+    assert torch.equal(p1.adam_m, p2.adam_m) and torch.equal(p1.adam_v, p2.adam_v)

@@ -48,19 +48,22 @@ def test_calib_entry_point_rejects_bad_arguments_without_a_gpu():
     lib = _native.lib()
     lr = (ctypes.c_float * 8)(*([0.1] * 8))
     assert lib.cbct_calib_workspace_bytes(None, 0) == 0
-    st = lib.cbct_calib_run(None, 0, 5, ctypes.addressof(lr), 0.9, 0.999, 1e-8, 0, None, 0, None)
+    st = lib.cbct_calib_run(None, 0, 5, 0, ctypes.addressof(lr), 1.0, 0.9, 0.999, 1e-8, 0, None, 0, None)
     assert st != 0 and b"at least one problem" in lib.cbct_last_error()
     prob = calibration._CalibProblem()
     prob.D = prob.H = prob.W = 8
     prob.A = prob.U = prob.V = 4
-    st = lib.cbct_calib_run(ctypes.addressof(prob), 1, -1, ctypes.addressof(lr), 0.9, 0.999, 1e-8, 0, None, 0,
-                            None)
+    st = lib.cbct_calib_run(ctypes.addressof(prob), 1, -1, 0, ctypes.addressof(lr), 1.0, 0.9, 0.999, 1e-8, 0,
+                            None, 0, None)
     assert st != 0 and b"nsteps" in lib.cbct_last_error()
-    st = lib.cbct_calib_run(ctypes.addressof(prob), 1, 3, ctypes.addressof(lr), 1.5, 0.999, 1e-8, 0, None, 0,
-                            None)
+    st = lib.cbct_calib_run(ctypes.addressof(prob), 1, 3, 0, ctypes.addressof(lr), 1.0, 1.5, 0.999, 1e-8, 0,
+                            None, 0, None)
     assert st != 0 and b"Adam" in lib.cbct_last_error()
     nbytes = lib.cbct_calib_workspace_bytes(ctypes.addressof(prob), 1)
     assert nbytes > 0
-    st = lib.cbct_calib_run(ctypes.addressof(prob), 1, 3, ctypes.addressof(lr), 0.9, 0.999, 1e-8, 0, None, 0,
-                            None)
+    st = lib.cbct_calib_run(ctypes.addressof(prob), 1, 3, 0, ctypes.addressof(lr), 1.5, 0.9, 0.999, 1e-8, 0,
+                            None, 0, None)
+    assert st != 0 and b"lr_decay" in lib.cbct_last_error()
+    st = lib.cbct_calib_run(ctypes.addressof(prob), 1, 3, 0, ctypes.addressof(lr), 1.0, 0.9, 0.999, 1e-8, 0,
+                            None, 0, None)
     assert st != 0 and b"workspace" in lib.cbct_last_error()

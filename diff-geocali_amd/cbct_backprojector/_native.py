@@ -51,6 +51,8 @@ _SIGNATURES = {
                              _F, _I64, _P, _P, _SZ, _P], _INT),
     "cbct_calib_workspace_bytes": ([_P, _INT], _SZ),
     "cbct_calib_run": ([_P, _INT, _INT, _INT, _P, _F, _F, _F, _F, _INT, _P, _SZ, _P], _INT),
+    "cbct_calib_lm_workspace_bytes": ([_P, _INT], _SZ),
+    "cbct_calib_run_lm": ([_P, _INT, _INT, _F, _INT, _P, _SZ, _P], _INT),
     # learned warm start (include/cbct_nn.h)
     "cbct_nn_gemm_workspace_bytes": ([_I64, _I64, _INT], _SZ),
     "cbct_nn_gemm": ([_INT, _INT, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _INT, _P, _I64, _P, _I64, _INT,

@@ -101,31 +101,45 @@ class CalibrationProblem:
 
 
 def calibrate(problems, steps=20, lr=DEFAULT_LR, betas=(0.9, 0.999), eps=1e-8, use_graph=True,
-              want_grad=False, lr_decay=1.0, first_step=0):
-    """Run ``steps`` Adam steps on every problem together; problems are updated in place
-    (``problem.geometry``, ``problem.adam_m/v``, ``problem.loss_history`` of length steps+1,
-    and with ``want_grad`` the loss gradient at the final geometry in ``problem.grad``).
-    Step t (counted from ``first_step``; continue a fit by passing the steps already taken)
-    uses the learning rates ``lr * lr_decay**t``."""
+              want_grad=False, lr_decay=1.0, first_step=0, solver="adam", lm_lambda=1e-3):
+    """Run ``steps`` optimisation steps on every problem together; problems are updated in
+    place (``problem.geometry``, ``problem.loss_history`` of length steps+1).
+
+    solver="adam": Adam on the loss gradient with per-group learning rates ``lr``; step t
+    (counted from ``first_step``; continue a fit by passing the steps already taken) uses
+    ``lr * lr_decay**t``; the moments live in ``problem.adam_m/v``; with ``want_grad`` the
+    gradient at the final geometry is returned in ``problem.grad``.
+
+    solver="lm": Levenberg-Marquardt per view on the Gauss-Newton normal equations formed by
+    the same fused march (initial damping ``lm_lambda``); ``loss_history`` holds the accepted
+    loss after each iteration and the geometry ends at the accepted iterate."""
     if isinstance(problems, CalibrationProblem):
         problems = [problems]
     problems = list(problems)
     if not problems:
         raise ValueError("no problems")
+    if solver not in ("adam", "lm"):
+        raise ValueError("solver must be 'adam' or 'lm'")
     device = problems[0].volume.device
     if any(p.volume.device != device for p in problems):
         raise ValueError("all problems must be on one device")
     lib = _native.lib()
-    arr = (_CalibProblem * len(problems))(*[p._struct(steps, want_grad) for p in problems])
+    lm = solver == "lm"
+    arr = (_CalibProblem * len(problems))(*[p._struct(steps, want_grad and not lm) for p in problems])
     lr_arr = (ctypes.c_float * 8)(*[float(x) for x in lr])
     with torch.cuda.device(device):
-        nbytes = lib.cbct_calib_workspace_bytes(ctypes.addressof(arr), len(problems))
+        size_fn = lib.cbct_calib_lm_workspace_bytes if lm else lib.cbct_calib_workspace_bytes
+        nbytes = size_fn(ctypes.addressof(arr), len(problems))
         ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
-        stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
-        status = lib.cbct_calib_run(ctypes.addressof(arr), len(problems), int(steps), int(first_step),
-                                    ctypes.addressof(lr_arr), float(lr_decay), float(betas[0]), float(betas[1]),
-                                    float(eps), 1 if use_graph else 0, ctypes.c_void_p(ws.data_ptr()), nbytes,
-                                    stream)
+        stream = torch.cuda.current_stream(device).cuda_stream
+        if lm:
+            status = lib.cbct_calib_run_lm(ctypes.addressof(arr), len(problems), int(steps), float(lm_lambda),
+                                           1 if use_graph else 0, ws.data_ptr(), nbytes, stream)
+        else:
+            status = lib.cbct_calib_run(ctypes.addressof(arr), len(problems), int(steps), int(first_step),
+                                        ctypes.addressof(lr_arr), float(lr_decay), float(betas[0]),
+                                        float(betas[1]), float(eps), 1 if use_graph else 0, ws.data_ptr(), nbytes,
+                                        stream)
     if status != 0:
         raise RuntimeError(lib.cbct_last_error().decode())
     return problems

@@ -47,7 +47,10 @@ __global__ void calib_frames_kernel(const CalibDev* __restrict__ probs, int npro
     const_cast<ViewFrame*>(d.s.frames)[a] = make_frame(gp, a);
 }
 
-// Per view: fixed-order sum of the tile partials (8 geometry gradients + loss).
+// Per view: fixed-order sum of the tile partials.  NP = 9 (Adam: 8 geometry gradients + loss,
+// written to ggeo / vloss and added to the loss history) or LM_NE (Levenberg-Marquardt: the
+// normal equations, written to ne; the history is written by the LM kernel).
+template <int NP>
 __global__ __launch_bounds__(256) void calib_reduce_kernel(const CalibDev* __restrict__ probs, int nprob,
                                                            int step) {
     const int gv = blockIdx.x;
@@ -55,29 +58,112 @@ __global__ __launch_bounds__(256) void calib_reduce_kernel(const CalibDev* __res
     const CalibDev& d = probs[p];
     const int a = gv - d.view_offset;
     const int tiles = d.s.tiles_u * d.s.tiles_v;
-    float acc[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int t = threadIdx.x; t < tiles; t += 256) {
-        const float* q = d.s.partials + ((int64_t)a * tiles + t) * 9;
+    float acc[NP];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) acc[k] += q[k];
+    for (int k = 0; k < NP; ++k) acc[k] = 0.f;
+    for (int t = threadIdx.x; t < tiles; t += 256) {
+        const float* q = d.s.partials + ((int64_t)a * tiles + t) * NP;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) acc[k] += q[k];
     }
-    __shared__ float sm[4][9];
+    __shared__ float sm[4][NP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
+    for (int k = 0; k < NP; ++k) {
         const float s = wave_sum(acc[k]);
         if (lane == 0) sm[wave][k] = s;
     }
     __syncthreads();
-    if (threadIdx.x < 9) {
+    if (threadIdx.x < NP) {
         const int k = threadIdx.x;
         const float s = (sm[0][k] + sm[1][k]) + (sm[2][k] + sm[3][k]);
-        if (k < 8) d.ggeo[k * d.A + a] = s;
-        else {
+        if (NP == LM_NE) {
+            d.ne[(int64_t)a * LM_NE + k] = s;
+            if (k == LM_NE - 1) d.vloss[a] = s;
+        } else if (k < 8) {
+            d.ggeo[k * d.A + a] = s;
+        } else {
             d.vloss[a] = s;
             atomicAdd(d.loss_hist + step, s);
         }
     }
+}
+
+// Levenberg-Marquardt step per view (one thread per view).  The march of this iteration gave
+// the normal equations at the current geometry; if its loss is not above the accepted one the
+// current geometry is accepted (damping / 3), otherwise the view returns to the accepted
+// geometry (damping x 4).  The next geometry is accepted + delta with
+//   (H + lambda diag(H)) delta = -J^T r,  H = J^T J,
+// solved by an FP64 Cholesky factorisation.  The loss history records the accepted loss.
+__global__ void calib_lm_kernel(const CalibDev* __restrict__ probs, int nprob, int total_views, int iter,
+                                float lambda0, int update) {
+    const int gv = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gv >= total_views) return;
+    const int p = find_problem(probs, nprob, gv, 0);
+    const CalibDev& d = probs[p];
+    const int a = gv - d.view_offset, A = d.A;
+    float* st = d.lm + (int64_t)a * LM_STATE;
+    const float* ne = d.ne + (int64_t)a * LM_NE;
+    const float loss = ne[LM_NE - 1];
+    float lambda;
+    if (iter == 0 || loss <= st[8 + LM_NE + 1]) {
+        for (int k = 0; k < 8; ++k) st[k] = d.geom[k * A + a];
+        for (int k = 0; k < LM_NE; ++k) st[8 + k] = ne[k];
+        lambda = iter == 0 ? lambda0 : fmaxf(st[8 + LM_NE] * (1.0f / 3.0f), 1e-7f);
+        st[8 + LM_NE + 1] = loss;
+    } else {
+        lambda = fminf(st[8 + LM_NE] * 4.0f, 1e7f);
+    }
+    st[8 + LM_NE] = lambda;
+    atomicAdd(d.loss_hist + iter, st[8 + LM_NE + 1]);
+    if (!update) {
+        for (int k = 0; k < 8; ++k) d.geom[k * A + a] = st[k];      // finish at the accepted geometry
+        return;
+    }
+    double M[8][8], b[8];
+    double dmax = 0.0;
+    {
+        int idx = 0;
+        for (int i = 0; i < 8; ++i)
+            for (int j = i; j < 8; ++j) {
+                M[i][j] = M[j][i] = (double)st[8 + idx];
+                ++idx;
+            }
+        for (int i = 0; i < 8; ++i) {
+            b[i] = -(double)st[8 + 36 + i];
+            dmax = fmax(dmax, M[i][i]);
+        }
+    }
+    const double floor_ = dmax > 0.0 ? 1e-12 * dmax : 1e-30;
+    for (int i = 0; i < 8; ++i) M[i][i] += (double)lambda * fmax(M[i][i], floor_) + (M[i][i] > 0.0 ? 0.0 : floor_);
+    // Cholesky M = L L^T (lower triangle in place)
+    bool ok = true;
+    for (int j = 0; j < 8 && ok; ++j) {
+        double s = M[j][j];
+        for (int k = 0; k < j; ++k) s -= M[j][k] * M[j][k];
+        if (!(s > 0.0)) { ok = false; break; }
+        const double l = sqrt(s);
+        M[j][j] = l;
+        for (int i = j + 1; i < 8; ++i) {
+            double t = M[i][j];
+            for (int k = 0; k < j; ++k) t -= M[i][k] * M[j][k];
+            M[i][j] = t / l;
+        }
+    }
+    double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (ok) {
+        for (int i = 0; i < 8; ++i) {            // L y = b
+            double t = b[i];
+            for (int k = 0; k < i; ++k) t -= M[i][k] * x[k];
+            x[i] = t / M[i][i];
+        }
+        for (int i = 7; i >= 0; --i) {           // L^T x = y
+            double t = x[i];
+            for (int k = i + 1; k < 8; ++k) t -= M[k][i] * x[k];
+            x[i] = t / M[i][i];
+        }
+    }
+    for (int k = 0; k < 8; ++k) d.geom[k * A + a] = (float)((double)st[k] + x[k]);
 }
 
 struct AdamArgs {
@@ -105,22 +191,25 @@ static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct CalibLayout {
     size_t descs, counter, total;
-    std::vector<size_t> frames, partials, ggeo, vloss;
+    std::vector<size_t> frames, partials, ggeo, vloss, ne, lm;
 };
 
 static int64_t tiles_of(int64_t U, int64_t V) { return ceil_div(U, SL_TILE) * ceil_div(V, SL_TILE); }
 
-static CalibLayout calib_layout(const cbct_calib_problem* pr, int n) {
+static CalibLayout calib_layout(const cbct_calib_problem* pr, int n, int solver) {
     CalibLayout L;
     size_t off = 0;
+    const size_t np = solver == 1 ? LM_NE : 9;
     L.descs = off; off += align256(sizeof(CalibDev) * (size_t)n);
     L.counter = off; off += 256;
     for (int i = 0; i < n; ++i) {
         const int64_t A = pr[i].A;
         L.frames.push_back(off); off += align256((size_t)A * sizeof(ViewFrame));
-        L.partials.push_back(off); off += align256((size_t)A * tiles_of(pr[i].U, pr[i].V) * 9 * sizeof(float));
+        L.partials.push_back(off); off += align256((size_t)A * tiles_of(pr[i].U, pr[i].V) * np * sizeof(float));
         L.ggeo.push_back(off); off += align256((size_t)A * 8 * sizeof(float));
         L.vloss.push_back(off); off += align256((size_t)A * sizeof(float));
+        L.ne.push_back(off); off += solver == 1 ? align256((size_t)A * LM_NE * sizeof(float)) : 0;
+        L.lm.push_back(off); off += solver == 1 ? align256((size_t)A * LM_STATE * sizeof(float)) : 0;
     }
     L.total = off;
     return L;
@@ -134,8 +223,17 @@ using namespace cbct;
 
 extern "C" size_t cbct_calib_workspace_bytes(const cbct_calib_problem* problems, int nproblems) {
     if (!problems || nproblems < 1) return 0;
-    return calib_layout(problems, nproblems).total;
+    return calib_layout(problems, nproblems, 0).total;
 }
+
+extern "C" size_t cbct_calib_lm_workspace_bytes(const cbct_calib_problem* problems, int nproblems) {
+    if (!problems || nproblems < 1) return 0;
+    return calib_layout(problems, nproblems, 1).total;
+}
+
+static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nsteps, int first_step, int solver,
+                           const float* lr8, float lr_decay, float beta1, float beta2, float eps, float lambda0,
+                           int use_graph, void* workspace, size_t workspace_bytes, void* stream_);
 
 extern "C" int cbct_calib_run(const cbct_calib_problem* pr, int nproblems, int nsteps, int first_step,
                               const float* lr8, float lr_decay, float beta1, float beta2, float eps, int use_graph,
@@ -147,8 +245,24 @@ extern "C" int cbct_calib_run(const cbct_calib_problem* pr, int nproblems, int n
     CBCT_CHECK_ARG(lr8 != nullptr, "lr must point to 8 learning rates");
     CBCT_CHECK_ARG(beta1 >= 0.f && beta1 < 1.f && beta2 >= 0.f && beta2 < 1.f && eps > 0.f,
                    "Adam parameters out of range");
+    return run_calibration(pr, nproblems, nsteps, first_step, 0, lr8, lr_decay, beta1, beta2, eps, 0.f, use_graph,
+                           workspace, workspace_bytes, stream_);
+}
+
+extern "C" int cbct_calib_run_lm(const cbct_calib_problem* pr, int nproblems, int nsteps, float lambda0,
+                                 int use_graph, void* workspace, size_t workspace_bytes, void* stream_) {
+    CBCT_CHECK_ARG(pr != nullptr && nproblems >= 1, "at least one problem is required");
+    CBCT_CHECK_ARG(nsteps >= 0 && nsteps <= 100000, "nsteps must be in [0, 100000]");
+    CBCT_CHECK_ARG(lambda0 > 0.f && lambda0 <= 1e6f, "lambda0 must be in (0, 1e6]");
+    return run_calibration(pr, nproblems, nsteps, 0, 1, nullptr, 1.f, 0.f, 0.f, 1.f, lambda0, use_graph,
+                           workspace, workspace_bytes, stream_);
+}
+
+static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nsteps, int first_step, int solver,
+                           const float* lr8, float lr_decay, float beta1, float beta2, float eps, float lambda0,
+                           int use_graph, void* workspace, size_t workspace_bytes, void* stream_) {
     hipStream_t stream = (hipStream_t)stream_;
-    const CalibLayout L = calib_layout(pr, nproblems);
+    const CalibLayout L = calib_layout(pr, nproblems, solver);
     CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= L.total, "workspace too small for cbct_calib_run");
     char* ws = (char*)workspace;
     int dflt[4] = {1, 8, 4096, 1};
@@ -157,7 +271,7 @@ extern "C" int cbct_calib_run(const cbct_calib_problem* pr, int nproblems, int n
     int total_tiles = 0, total_views = 0, total_params = 0, max_cap = 256;
     for (int i = 0; i < nproblems; ++i) {
         const cbct_calib_problem& q = pr[i];
-        CBCT_CHECK_ARG(q.volume && q.measured && q.geometry && q.adam_m && q.adam_v && q.loss_history,
+        CBCT_CHECK_ARG(q.volume && q.measured && q.geometry && q.loss_history && (solver == 1 || (q.adam_m && q.adam_v)),
                        "problem pointers must be non-null");
         CBCT_CHECK_ARG(q.D > 0 && q.H > 0 && q.W > 0 && q.D * q.H * q.W <= INT32_MAX,
                        "volume dimensions must be positive and within the 2^31-1 element limit");
@@ -191,6 +305,9 @@ extern "C" int cbct_calib_run(const cbct_calib_problem* pr, int nproblems, int n
         d.loss_hist = q.loss_history;
         d.A = (int)q.A;
         d.div = is_pow2(q.voxel_size) ? 0 : 1;
+        d.solver = solver;
+        d.ne = solver == 1 ? (float*)(ws + L.ne[i]) : nullptr;
+        d.lm = solver == 1 ? (float*)(ws + L.lm[i]) : nullptr;
         d.view_offset = total_views;
         d.tile_offset = total_tiles;
         d.param_offset = total_params;
@@ -215,7 +332,7 @@ extern "C" int cbct_calib_run(const cbct_calib_problem* pr, int nproblems, int n
     const int grid = std::max(1, std::min(total_tiles, cus * 8));
     (void)per_cu;
     AdamArgs h;
-    for (int k = 0; k < 8; ++k) h.lr[k] = lr8[k];
+    for (int k = 0; k < 8; ++k) h.lr[k] = lr8 ? lr8[k] : 0.f;
     h.beta1 = beta1; h.beta2 = beta2; h.eps = eps;
 
     auto enqueue = [&](int step, bool update, hipStream_t st_) -> int {
@@ -225,7 +342,15 @@ extern "C" int cbct_calib_run(const cbct_calib_problem* pr, int nproblems, int n
         CBCT_LAUNCHED("calibration frames kernel launch");
         int st;
         if ((st = launch_fp_slab_batched(ddev, nproblems, total_tiles, max_cap, counter, grid, st_))) return st;
-        calib_reduce_kernel<<<(unsigned)total_views, 256, 0, st_>>>(ddev, nproblems, step);
+        if (solver == 1) {
+            calib_reduce_kernel<LM_NE><<<(unsigned)total_views, 256, 0, st_>>>(ddev, nproblems, step);
+            CBCT_LAUNCHED("calibration reduce kernel launch");
+            calib_lm_kernel<<<(unsigned)ceil_div(total_views, 64), 64, 0, st_>>>(ddev, nproblems, total_views, step,
+                                                                                 lambda0, update ? 1 : 0);
+            CBCT_LAUNCHED("calibration LM kernel launch");
+            return CBCT_OK;
+        }
+        calib_reduce_kernel<9><<<(unsigned)total_views, 256, 0, st_>>>(ddev, nproblems, step);
         CBCT_LAUNCHED("calibration reduce kernel launch");
         if (update) {
             const int t = first_step + step;

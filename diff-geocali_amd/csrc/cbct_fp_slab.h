@@ -35,7 +35,13 @@ struct CalibDev {
     float* loss_hist;    // [nsteps] loss before each update
     int A, view_offset, tile_offset, param_offset;
     int div;             // voxel size not a power of two: the forward's division map
+    int solver;          // 0: Adam on the gradient; 1: Levenberg-Marquardt on the normal equations
+    float* ne;           // [A][45] normal equations of the current geometry (solver 1)
+    float* lm;           // [A][LM_STATE] accepted geometry, its normal equations, damping, loss
 };
+
+constexpr int LM_NE = 45;               // 36 (upper triangle of J^T J) + 8 (J^T r) + loss
+constexpr int LM_STATE = 8 + LM_NE + 2; // geometry, normal equations, lambda, loss
 
 // mode 0: forward; 1: adjoint (geometry only); 2: adjoint with the volume gradient.
 int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream);

@@ -43,6 +43,9 @@ __device__ __forceinline__ int box_index_div(unsigned i, unsigned d, unsigned ma
 // 3: calibration step -- forward, squared residual against the measured projections (P.grad) and
 //    the geometry gradient of the loss, in one march (the geometry sums are linear in the upstream
 //    gradient, so they are accumulated with unit weight and scaled by the residual at the end).
+// 4: Gauss-Newton step -- as 3, but each ray's eight geometry derivatives j of its projection and
+//    its residual r give the tile's normal equations: the 36 entries of sum j j^T (upper triangle,
+//    row-major), the 8 of sum j r and the loss (45 partials per tile).
 template <int MODE, bool DIV>
 __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     extern __shared__ float lds[];
@@ -70,12 +73,12 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     float go = 0.f;
     if (inr) {
         if (MODE != 0) go = P.grad[((int64_t)a * P.U + u) * P.V + v];
-        if ((MODE == 0 || MODE == 3 || go != 0.f) && make_ray(f, u, v, P.pix, ray)) {
+        if ((MODE == 0 || MODE >= 3 || go != 0.f) && make_ray(f, u, v, P.pix, ray)) {
             valid = true;
             if (intersect(b, f, ray, tmin, tmax, entry)) { hit = true; alive = true; t = tmin; }
         }
     }
-    const float gs = MODE == 3 ? 1.0f : go * P.step;
+    const float gs = MODE >= 3 ? 1.0f : go * P.step;
     float integral = 0.f;
     // Fixed-point scale of the LDS gradient box (MODE 2): a power of two 2^e chosen so that the
     // largest total a voxel can receive from this tile within one slab stays below 2^30.  That
@@ -129,7 +132,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
 
     auto voxel_coords = [&](float& x, float& y, float& z) {
         const float px = fmaf(t, ray.dx, f.ox), py = fmaf(t, ray.dy, f.oy), pz = fmaf(t, ray.dz, f.oz);
-        if (MODE == 0 || MODE == 3) {      // the forward's voxel map (division unless vs is a power of two)
+        if (MODE == 0 || MODE >= 3) {      // the forward's voxel map (division unless vs is a power of two)
             x = to_voxel<DIV>(px, b.offx, P.vs, P.inv_vs, b.hx);
             y = to_voxel<DIV>(py, b.offy, P.vs, P.inv_vs, b.hy);
             z = to_voxel<DIV>(pz, b.offz, P.vs, P.inv_vs, b.hz);
@@ -141,7 +144,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     };
     // Everything a sample does once its eight taps are known.
     auto use_taps = [&](const Taps& tp, float fx, float fy, float fz) {
-        if (MODE == 0 || MODE == 3) integral += tri_value(tp, tri_weights(fx, fy, fz));
+        if (MODE == 0 || MODE >= 3) integral += tri_value(tp, tri_weights(fx, fy, fz));
         if (MODE != 0) {
             float sgx, sgy, sgz;
             tri_gradient(tp, fx, fy, fz, sgx, sgy, sgz);
@@ -347,7 +350,14 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         if (inr) P.proj[((int64_t)a * P.U + u) * P.V + v] = hit ? integral * P.step : 0.f;
         return;
     }
-    float loss = 0.f;
+    float loss = 0.f, resid = 0.f;
+    if (MODE == 4 && inr) {
+        // unit-weight sums times step: the derivatives of this ray's projection value
+        resid = (hit ? integral * P.step : 0.f) - go;
+        loss = 0.5f * P.resid_scale * resid * resid;
+        gox *= P.step; goy *= P.step; goz *= P.step; gdx *= P.step; gdy *= P.step; gdz *= P.step;
+        gtmin *= P.step;
+    }
     if (MODE == 3 && inr) {
         // residual of this ray; scale the unit-weight geometry sums by d(loss)/d(proj) * step
         const float r = (hit ? integral * P.step : 0.f) - go;
@@ -377,6 +387,36 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         acc[3] = -P.pix * fmaf(gry, f.s, grx * f.c);
         acc[4] = -P.pix * grz;
         acc[5] = Ox; acc[6] = Oy; acc[7] = Oz;
+    }
+    if (MODE == 4) {
+        // normal equations of the tile: upper triangle of sum j j^T, sum j r, loss
+        constexpr int NE = 45;
+        __shared__ float red4[SL_THREADS / 64][NE];
+        int idx = 0;
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+#pragma unroll
+            for (int q = p; q < 8; ++q) {
+                const float s = wave_sum(acc[p] * acc[q]);
+                if (lane == 0) red4[wave][idx] = s;
+                ++idx;
+            }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float s = wave_sum(acc[k] * resid);
+            if (lane == 0) red4[wave][36 + k] = s;
+        }
+        {
+            const float s = wave_sum(loss);
+            if (lane == 0) red4[wave][44] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < NE) {
+            const int k = threadIdx.x;
+            const float s = (red4[0][k] + red4[1][k]) + (red4[2][k] + red4[3][k]);
+            P.partials[((int64_t)a * P.tiles_u * P.tiles_v + tu * P.tiles_v + tv) * NE + k] = s;
+        }
+        return;
     }
     constexpr int NP = MODE == 3 ? 9 : 8;
     __shared__ float red[SL_THREADS / 64][9];
@@ -423,8 +463,14 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_batched_kernel(const Calib
             if (probs[mid].tile_offset <= tile) lo = mid; else hi = mid - 1;
         }
         const SlabArgs P = probs[lo].s;
-        if (probs[lo].div) slab_tile<3, true>(P, tile - probs[lo].tile_offset);
-        else slab_tile<3, false>(P, tile - probs[lo].tile_offset);
+        const int t = tile - probs[lo].tile_offset;
+        if (probs[lo].solver == 1) {
+            if (probs[lo].div) slab_tile<4, true>(P, t);
+            else slab_tile<4, false>(P, t);
+        } else {
+            if (probs[lo].div) slab_tile<3, true>(P, t);
+            else slab_tile<3, false>(P, t);
+        }
     }
 }
 

@@ -189,6 +189,19 @@ int cbct_calib_run(const cbct_calib_problem* problems, int nproblems, int nsteps
                    const float* lr8, float lr_decay, float beta1, float beta2, float eps, int use_graph,
                    void* workspace, size_t workspace_bytes, void* stream);
 
+/* The same batched calibration with a Levenberg-Marquardt solver per view: each
+ * iteration's fused march also forms the view's Gauss-Newton normal equations
+ * (J^T J of the eight geometry derivatives of every ray's projection, J^T r
+ * and the loss), and an 8x8 damped solve in FP64 gives the next geometry; an
+ * iteration whose loss rises is undone for that view and retried with more
+ * damping.  lambda0 = initial damping (relative to diag(J^T J)).  adam_m and
+ * adam_v may be NULL.  loss_history[i] = the accepted loss after i iterations;
+ * the geometry ends at the accepted iterate.  No counterpart in the reference
+ * (its calibration loop is left to user code). */
+size_t cbct_calib_lm_workspace_bytes(const cbct_calib_problem* problems, int nproblems);
+int cbct_calib_run_lm(const cbct_calib_problem* problems, int nproblems, int nsteps, float lambda0,
+                      int use_graph, void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -134,3 +134,63 @@ def test_continued_fit_equals_one_long_fit():
     calibration.calibrate([p2], steps=3, use_graph=True, lr_decay=0.9, first_step=2)
     assert torch.equal(p1.geometry, p2.geometry)
     assert torch.equal(p1.adam_m, p2.adam_m) and torch.equal(p1.adam_v, p2.adam_v)
+
+
+def test_lm_step_solves_the_gauss_newton_normal_equations():
+    """One Levenberg-Marquardt iteration with negligible damping: the geometry change delta
+    satisfies H delta = -g, with H = sum j j^T and g = sum j r built in FP64 from per-ray
+    derivatives j of the plain adjoint (one-hot upstream gradients) and residuals r of the
+    plain forward projection."""
+    spec = dict(SMALL, D=24, A=2, U=8, V=8, seed=17)
+    spec["pixel"] = (1100 / 600) * 1.42 * 24 * 0.5 / (0.95 * 8)
+    p = _problem(spec)
+    g0 = p.geometry.clone()
+    A, U, V = p.measured.shape
+    glist = [g.contiguous() for g in g0]
+    proj = _native.fp_forward(p.volume, glist, p.pixel_size, U, V, p.step_size, p.voxel_size, 0., 0., 0.)
+    r = (proj - p.measured).double().cpu()
+    J = torch.zeros((A, U, V, 8), dtype=torch.float64)
+    for u in range(U):
+        for v in range(V):
+            up = torch.zeros((A, U, V), device=DEV)
+            up[:, u, v] = 1.0
+            _, gg = _native.fp_backward(up, p.volume, glist, p.pixel_size, p.step_size, p.voxel_size, False,
+                                        0., 0., 0.)
+            J[:, u, v, :] = torch.stack(list(gg)).double().cpu().T
+    calibration.calibrate([p], steps=1, solver="lm", lm_lambda=1e-6, use_graph=False)
+    delta = (p.geometry.double() - g0.double()).cpu()
+    assert delta.abs().max() > 0, "the first iteration was rejected"
+    for a in range(A):
+        Ja = J[a].reshape(-1, 8)
+        H = Ja.T @ Ja
+        g = Ja.T @ r[a].reshape(-1)
+        res = (H @ delta[:, a] + g).norm() / g.norm()
+        print(f"view {a}: |H delta + g| / |g| = {res:.2e}")
+        assert res <= 1e-2
+
+
+def test_lm_converges_monotonically_and_beats_adam_per_iteration():
+    spec = dict(SMALL, D=48, A=24, U=64, V=64, seed=21)
+    spec["pixel"] = (1100 / 600) * 1.42 * 48 * 0.5 / (0.95 * 64)
+    p_lm, p_adam = _problem(spec), _problem(spec)
+    shape = tuple(p_lm.volume.shape)
+    calibration.calibrate([p_lm], steps=10, solver="lm", use_graph=True)
+    calibration.calibrate([p_adam], steps=10, use_graph=True)
+    hist = p_lm.loss_history.cpu().numpy()
+    assert np.all(np.diff(hist) <= 0)
+    e_lm = calibration.reprojection_error_px(p_lm.geometry, p_lm.true_geometry, shape, p_lm.voxel_size,
+                                             p_lm.pixel_size)
+    e_adam = calibration.reprojection_error_px(p_adam.geometry, p_adam.true_geometry, shape, p_adam.voxel_size,
+                                               p_adam.pixel_size)
+    print(f"10 iterations: LM {e_lm:.3f} px (loss {hist[0]:.2e} -> {hist[-1]:.2e}), Adam {e_adam:.3f} px")
+    assert e_lm < e_adam and e_lm < 0.2
+
+
+def test_lm_batched_problems_equal_individual_runs():
+    specs = [dict(SMALL), dict(SMALL, D=33, A=11, U=40, V=40, seed=9)]
+    batch = [_problem(s) for s in specs]
+    calibration.calibrate(batch, steps=3, solver="lm", use_graph=True)
+    for s, pb in zip(specs, batch):
+        pa = _problem(s)
+        calibration.calibrate([pa], steps=3, solver="lm", use_graph=False)
+        assert torch.equal(pa.geometry, pb.geometry)

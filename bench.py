@@ -40,6 +40,9 @@ WORKLOADS = {
 CALIB_WORKLOAD = "batched-calibration-64p"
 WORKLOADS[CALIB_WORKLOAD] = dict(problems=64, adam_steps=20, side=(96, 320), views=(90, 720), detector=(128, 768),
                                  voxel=0.5, sod=600.0, sid=1100.0, seed=4)
+WARM_WORKLOAD = "calibration-time-256c-180v"
+WORKLOADS[WARM_WORKLOAD] = dict(train_steps=1500, pool=24, batch=2, scans=(7001, 7002, 7003, 7004), threshold=0.1,
+                                adam_budget=150, lm_budget=20, plain_adam_lr=4.0, warm_adam_lr=1.0, lm_lambda=1e-3)
 DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 PHANTOM_SEED = 20261203
 PERTURB_SEED = 7
@@ -57,6 +60,10 @@ def parse():
                    help="run a 1 GiB reference copy, one forward and one adjoint (for rocprofv3 --pmc runs) and exit")
     p.add_argument("--adam-steps", type=int, default=None,
                    help="Adam steps per problem of the batched-calibration workload (default 20)")
+    p.add_argument("--solver", default="adam", choices=("adam", "lm"),
+                   help="optimiser of the batched-calibration workload (the workload is defined with Adam)")
+    p.add_argument("--train-steps", type=int, default=None,
+                   help="training steps of the warm-start network in the calibration-time workload")
     p.add_argument("--fp-tuning", default=None,
                    help="projector kernels as use_slab,slab_thickness,lds_floats,layout (default: library's)")
     return p.parse_args()
@@ -163,7 +170,7 @@ def calibration_bench(args, torch, device):
     for _ in range(args.warmup):
         reset()
         t0 = time.perf_counter()
-        calibration.calibrate(probs, steps=nsteps, use_graph=True)
+        calibration.calibrate(probs, steps=nsteps, use_graph=True, solver=args.solver)
         print(f"# warm-up calibration {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     times = []
     t_wall = 0.0
@@ -173,7 +180,7 @@ def calibration_bench(args, torch, device):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
         e0.record()
-        calibration.calibrate(probs, steps=nsteps, use_graph=True)
+        calibration.calibrate(probs, steps=nsteps, use_graph=True, solver=args.solver)
         e1.record()
         torch.cuda.synchronize()
         t_wall += time.perf_counter() - t0
@@ -195,7 +202,7 @@ def calibration_bench(args, torch, device):
         "metric": METRIC, "value": updates / t / 1e9, "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": t_wall / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": CALIB_WORKLOAD, "problems": len(probs), "adam_steps": nsteps,
+        "config": {"workload": CALIB_WORKLOAD, "problems": len(probs), "solver": args.solver, "steps_per_problem": nsteps,
                    "s_per_batched_calibration": t, "samples_per_march": samples, "rays": rays,
                    "voxel_view_updates_per_march": updates // marches,
                    "setup_s": setup_s, "autotune_s": tune_s,
@@ -204,6 +211,122 @@ def calibration_bench(args, torch, device):
                    "tunings": sorted({str(p.tuning) for p in probs})},
         "roofline": {"bound": "hbm", "kernel": "fused calibration march (whole graph timed)", "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def warm_start_bench(args, torch, device):
+    """Seconds to 0.1 px reprojection geometry error on 256^3 x 180-view scans (detector
+    512^2, pixel 0.8 mm, geometry perturbed as in the 256^3 x 720 workload), learned warm start
+    vs plain start.  The network is first trained here on projector-generated scans (its
+    training time is reported, not counted).  For each held-out scan and each method the number
+    of iterations to reach the threshold is found first, untimed; then exactly that many
+    iterations run again from the same start inside one timed call (warm start included for
+    the learned methods).  Methods: plain Adam, learned + Adam, plain Levenberg-Marquardt,
+    learned + Levenberg-Marquardt."""
+    from cbct_backprojector import calibration as C
+    from cbct_backprojector import warmstart as ws
+    w = WORKLOADS[WARM_WORKLOAD]
+    steps_train = w["train_steps"] if args.train_steps is None else args.train_steps
+    fam = ws.ScanFamily()
+    log = lambda m: print(f"# {m}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    data = ws.TrainingData(fam, pool=w["pool"], seed=1000, device=device)
+    net = ws.WarmStartNet(seed=0, device=device)
+    hist = ws.train(net, data, steps_train, P=w["batch"], lr=1e-3, lr_final=1e-5, log_every=250, log=log)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del data
+    thr = w["threshold"]
+    err = lambda p: C.reprojection_error_px(p.geometry, p.true_geometry, tuple(p.volume.shape), p.voxel_size,
+                                            p.pixel_size)
+
+    def reset(p, g):
+        p.geometry.copy_(g)
+        p.adam_m.zero_()
+        p.adam_v.zero_()
+
+    def adam_iters(p, g, mult):
+        reset(p, g)
+        lr = [x * mult for x in C.DEFAULT_LR]
+        for t in range(w["adam_budget"]):
+            C.calibrate([p], steps=1, lr=lr, first_step=t, use_graph=False)
+            if err(p) < thr:
+                return t + 1, err(p)
+        return None, err(p)
+
+    def lm_iters(p, g):
+        for t in range(1, w["lm_budget"] + 1):
+            reset(p, g)
+            C.calibrate([p], steps=t, solver="lm", lm_lambda=w["lm_lambda"])
+            if err(p) < thr:
+                return t, err(p)
+        return None, err(p)
+
+    def timed(p, g_start, n, solver, mult, warm):
+        reset(p, g_start)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if warm:
+            ws.warm_start(p, net, fam, iterations=warm)
+        if n:
+            if solver == "lm":
+                C.calibrate([p], steps=n, solver="lm", lm_lambda=w["lm_lambda"])
+            else:
+                C.calibrate([p], steps=n, lr=[x * mult for x in C.DEFAULT_LR])
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, err(p)
+
+    rows = []
+    for seed in w["scans"]:
+        p = C.synthetic_problem(fam.spec(seed), device)
+        g_nom = p.geometry.clone()
+        e_nom = err(p)
+        g_warm, e_warm = {}, {}
+        for k in (1, 2, 3):            # refinement passes of the network
+            reset(p, g_nom)
+            ws.warm_start(p, net, fam, iterations=k)
+            g_warm[k], e_warm[k] = p.geometry.clone(), err(p)
+        row = {"scan": seed, "px_nominal": e_nom, "px_warm_start": e_warm}
+        methods = [("plain_adam", g_nom, "adam", w["plain_adam_lr"], 0),
+                   ("learned_adam", g_warm[3], "adam", w["warm_adam_lr"], 3), ("plain_lm", g_nom, "lm", None, 0)]
+        methods += [(f"learned{k}_lm", g_warm[k], "lm", None, k) for k in (1, 2, 3)]
+        for name, start, solver, mult, warm in methods:
+            n, e_end = adam_iters(p, start, mult) if solver == "adam" else lm_iters(p, start)
+            budget = w["adam_budget"] if solver == "adam" else w["lm_budget"]
+            t, e_t = timed(p, g_nom, n if n else budget, solver, mult, warm)
+            row[name] = {"iterations": n, "s": t if n else None, "px": e_t,
+                         "s_budget_unreached": None if n else t}
+            log(f"scan {seed} {name}: {n} iterations, {t:.3f} s, {e_t:.3f} px")
+        rows.append(row)
+
+    def med(key):
+        v = [r[key]["s"] for r in rows]
+        return None if any(x is None for x in v) else sorted(v)[len(v) // 2]
+
+    t_plain_adam, t_learned_adam = med("plain_adam"), med("learned_adam")
+    t_plain_lm = med("plain_lm")
+    learned_lm = {k: med(f"learned{k}_lm") for k in (1, 2, 3)}
+    best_k = min((k for k in learned_lm if learned_lm[k] is not None), key=lambda k: learned_lm[k], default=3)
+    t_learned_lm = learned_lm[best_k]
+    ratio = lambda a, b: None if a is None or b is None else a / b
+    out = {
+        "metric": METRIC, "value": t_learned_lm, "unit": "s to 0.1 px (median over scans)", "n_gpus": 1,
+        "steps": len(rows), "warmup": 0, "ms_per_step": None, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 projector / bf16 network", "data": "synthetic",
+        "config": {"workload": WARM_WORKLOAD, "volume": [fam.D] * 3, "views": fam.A, "detector": [fam.U, fam.V],
+                   "pixel_mm": fam.pixel, "threshold_px": thr, "train_steps": steps_train, "train_s": train_s,
+                   "train_loss_first_last": [sum(hist[:50]) / 50, sum(hist[-50:]) / 50],
+                   "network_parameters": net.num_parameters,
+                   "median_s": {"plain_adam": t_plain_adam, "learned_adam": t_learned_adam,
+                                "plain_lm": t_plain_lm, "learned_lm": t_learned_lm,
+                                "learned_lm_by_network_passes": learned_lm},
+                   "network_passes": best_k,
+                   "speedup_learned_lm_vs_plain_adam": ratio(t_plain_adam, t_learned_lm),
+                   "speedup_learned_lm_vs_plain_lm": ratio(t_plain_lm, t_learned_lm),
+                   "speedup_learned_adam_vs_plain_adam": ratio(t_plain_adam, t_learned_adam),
+                   "adam_lr_multipliers": {"plain": w["plain_adam_lr"], "learned": w["warm_adam_lr"]},
+                   "scans": rows},
     }
     print(json.dumps(out), flush=True)
 
@@ -223,6 +346,8 @@ def main():
     torch.cuda.set_device(device)
     if args.workload == CALIB_WORKLOAD:
         return calibration_bench(args, torch, device)
+    if args.workload == WARM_WORKLOAD:
+        return warm_start_bench(args, torch, device)
     if args.fp_tuning:
         _native.set_fp_tuning(*[int(x) for x in args.fp_tuning.split(",")])
     w = WORKLOADS[args.workload]

@@ -44,6 +44,11 @@ WARM_WORKLOAD = "calibration-time-256c-180v"
 WORKLOADS[WARM_WORKLOAD] = dict(train_steps=1500, pool=24, batch=2, scans=(7001, 7002, 7003, 7004), threshold=0.1,
                                 adam_budget=150, lm_budget=20, plain_adam_lr=4.0, warm_adam_lr=1.0, lm_lambda=1e-3)
 DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
+# Physical HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs of
+# `bench.py --pmc-probe` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB copy in the
+# same runs): adjoint fp_slab_kernel<2> at 256^3 x 720 views, 768x640 detector.
+MEASURED_TRAFFIC = {("geometry-gradients-256c-720v", "adjoint"): 95_657_444 * 1024 * 2.0 + 134_343_710 * 1024,
+                    ("geometry-gradients-256c-720v", "forward"): 111_037_023 * 1024 * 2.0 + 1_508_828 * 1024}
 PHANTOM_SEED = 20261203
 PERTURB_SEED = 7
 
@@ -64,6 +69,8 @@ def parse():
                    help="optimiser of the batched-calibration workload (the workload is defined with Adam)")
     p.add_argument("--train-steps", type=int, default=None,
                    help="training steps of the warm-start network in the calibration-time workload")
+    p.add_argument("--scans", type=int, default=None,
+                   help="held-out scans of the calibration-time workload (default 4)")
     p.add_argument("--fp-tuning", default=None,
                    help="projector kernels as use_slab,slab_thickness,lds_floats,layout (default: library's)")
     return p.parse_args()
@@ -278,7 +285,7 @@ def warm_start_bench(args, torch, device):
         return time.perf_counter() - t0, err(p)
 
     rows = []
-    for seed in w["scans"]:
+    for seed in w["scans"][:args.scans]:
         p = C.synthetic_problem(fam.spec(seed), device)
         g_nom = p.geometry.clone()
         e_nom = err(p)
@@ -435,7 +442,8 @@ def main():
                    "t_fwd_ms": t_fwd * 1e3, ("t_fdk_ms" if fdk else "t_adj_ms"): t_adj * 1e3,
                    "fp_tuning": list(_native.get_fp_tuning())},
         "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": None if args.fp_tuning else MEASURED_TRAFFIC.get((args.workload, dominant[0])),
                      "pair_frac": (b_fwd + b_adj) / (t_fwd + t_adj) / 1e9 / HBM_PEAK_GBPS,
                      "copy_GBps": copy_bw},
     }

@@ -43,6 +43,8 @@ WORKLOADS[CALIB_WORKLOAD] = dict(problems=64, adam_steps=20, side=(96, 320), vie
 WARM_WORKLOAD = "calibration-time-256c-180v"
 WORKLOADS[WARM_WORKLOAD] = dict(train_steps=1500, pool=24, batch=2, scans=(7001, 7002, 7003, 7004), threshold=0.1,
                                 adam_budget=150, lm_budget=20, plain_adam_lr=4.0, warm_adam_lr=1.0, lm_lambda=1e-3)
+PREP_WORKLOAD = "preprocess-720v-768x640-u16"
+WORKLOADS[PREP_WORKLOAD] = dict(A=720, U=768, V=640)
 DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 # Physical HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs of
 # `bench.py --pmc-probe` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB copy in the
@@ -69,6 +71,8 @@ def parse():
                    help="optimiser of the batched-calibration workload (the workload is defined with Adam)")
     p.add_argument("--train-steps", type=int, default=None,
                    help="training steps of the warm-start network in the calibration-time workload")
+    p.add_argument("--deterministic", action="store_true",
+                   help="bitwise-reproducible volume gradient (int64 fixed-point accumulation)")
     p.add_argument("--scans", type=int, default=None,
                    help="held-out scans of the calibration-time workload (default 4)")
     p.add_argument("--fp-tuning", default=None,
@@ -338,6 +342,43 @@ def warm_start_bench(args, torch, device):
     print(json.dumps(out), flush=True)
 
 
+def preprocess_bench(args, torch, device):
+    """Raw uint16 frames [720, 640, 768] ([A,V,U]) + dark/flat -> line integrals [720, 768, 640]
+    fp32 (dark/flat correction, -log, transpose) in one kernel; HBM-bound.  Algorithmic bytes:
+    2 B read + 4 B written per pixel, plus the two fp32 fields once."""
+    from cbct_backprojector import formats
+    w = WORKLOADS[PREP_WORKLOAD]
+    A, U, V = w["A"], w["U"], w["V"]
+    gen = torch.Generator(device=device).manual_seed(3)
+    raw = torch.randint(200, 60000, (A, V, U), generator=gen, device=device, dtype=torch.int32).to(torch.uint16)
+    dark = torch.full((V, U), 100.0, device=device)
+    flat = torch.full((V, U), 60000.0, device=device)
+    for _ in range(args.warmup):
+        formats.preprocess(raw, dark, flat)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        out = formats.preprocess(raw, dark, flat)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t = e0.elapsed_time(e1) / args.steps / 1e3
+    nbytes = A * U * V * 6 + 2 * U * V * 4
+    achieved = nbytes / t / 1e9
+    del out
+    print(json.dumps({
+        "metric": METRIC, "value": A * U * V / t / 1e9, "unit": "Gpixel/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u16->f32", "data": "synthetic",
+        "config": {"workload": PREP_WORKLOAD, "views": A, "detector": [U, V], "layout_in": "AVU",
+                   "ms_per_pass": t * 1e3},
+        "roofline": {"bound": "hbm", "kernel": "preprocess_kernel<uint16_t,1>", "achieved": achieved,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
+    }), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -355,8 +396,12 @@ def main():
         return calibration_bench(args, torch, device)
     if args.workload == WARM_WORKLOAD:
         return warm_start_bench(args, torch, device)
+    if args.workload == PREP_WORKLOAD:
+        return preprocess_bench(args, torch, device)
     if args.fp_tuning:
         _native.set_fp_tuning(*[int(x) for x in args.fp_tuning.split(",")])
+    if args.deterministic:
+        _native.set_deterministic(True)
     w = WORKLOADS[args.workload]
     vol_np, geom_np, vol, geom, upstream = make_inputs(w, device)
     D, A, U, V = w["D"], w["A"], w["U"], w["V"]
@@ -440,10 +485,11 @@ def main():
                    "pixel_mm": w["pixel"], "sod_mm": w["sod"], "sid_mm": w["sid"], "voxel_mm": w["voxel"],
                    "step_mm": w["step"], "geometry_perturbed": w["perturb"], "samples": S,
                    "t_fwd_ms": t_fwd * 1e3, ("t_fdk_ms" if fdk else "t_adj_ms"): t_adj * 1e3,
-                   "fp_tuning": list(_native.get_fp_tuning())},
+                   "fp_tuning": list(_native.get_fp_tuning()), "deterministic": args.deterministic},
         "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": None if args.fp_tuning else MEASURED_TRAFFIC.get((args.workload, dominant[0])),
+                     "traffic": None if (args.fp_tuning or args.deterministic)
+                     else MEASURED_TRAFFIC.get((args.workload, dominant[0])),
                      "pair_frac": (b_fwd + b_adj) / (t_fwd + t_adj) / 1e9 / HBM_PEAK_GBPS,
                      "copy_GBps": copy_bw},
     }

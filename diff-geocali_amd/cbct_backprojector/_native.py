@@ -39,6 +39,9 @@ _SIGNATURES = {
     "cbct_fp_count_samples": ([_I64, _I64, _I64] + [_P] * 8 + [_I64, _F, _I64, _I64, _F, _F, _F, _F, _F,
                               _P, _P, _SZ, _P], _INT),
     "cbct_fp_set_tuning": ([_INT, _INT, _INT, _INT], _INT),
+    "cbct_fp_backward_workspace_bytes": ([_I64, _I64, _I64, _I64, _I64, _I64], _SZ),
+    "cbct_fp_set_deterministic": ([_INT], _INT),
+    "cbct_fp_get_deterministic": ([], _INT),
     "cbct_fp_get_tuning": ([_P], _INT),
     "cbct_bp_workspace_bytes": ([_I64, _I64, _I64, _I64], _SZ),
     "cbct_bp_forward": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
@@ -53,6 +56,7 @@ _SIGNATURES = {
     "cbct_calib_run": ([_P, _INT, _INT, _INT, _P, _F, _F, _F, _F, _INT, _P, _SZ, _P], _INT),
     "cbct_calib_lm_workspace_bytes": ([_P, _INT], _SZ),
     "cbct_calib_run_lm": ([_P, _INT, _INT, _F, _INT, _P, _SZ, _P], _INT),
+    "cbct_preprocess": ([_P, _INT, _P, _P, _I64, _I64, _I64, _INT, _F, _INT, _P, _P], _INT),
     # learned warm start (include/cbct_nn.h)
     "cbct_nn_gemm_workspace_bytes": ([_I64, _I64, _INT], _SZ),
     "cbct_nn_gemm": ([_INT, _INT, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _INT, _P, _I64, _P, _I64, _INT,
@@ -129,6 +133,31 @@ def get_fp_tuning():
     return tuple(out)
 
 
+def set_deterministic(on):
+    """Bitwise-reproducible projector volume gradients (int64 fixed-point accumulation)."""
+    _check(lib().cbct_fp_set_deterministic(1 if on else 0))
+
+
+def get_deterministic():
+    return bool(lib().cbct_fp_get_deterministic())
+
+
+class deterministic:
+    """Context manager for the deterministic volume-gradient mode."""
+
+    def __init__(self, on=True):
+        self.new = bool(on)
+
+    def __enter__(self):
+        self.old = get_deterministic()
+        set_deterministic(self.new)
+        return self
+
+    def __exit__(self, *exc):
+        set_deterministic(self.old)
+        return False
+
+
 class fp_tuning:
     """Context manager that applies a projector tuning and restores the previous one."""
 
@@ -174,7 +203,7 @@ def fp_backward(grad, volume, geometry, pixel_size, step_size, voxel_size, compu
         gvol = (torch.empty((D, H, W), dtype=torch.float32, device=dev) if compute_grad_volume
                 else torch.empty((0,), dtype=torch.float32, device=dev))
         ggeo = torch.empty((8, A), dtype=torch.float32, device=dev)
-        ws_bytes = L.cbct_fp_workspace_bytes(A, U, V)
+        ws_bytes = L.cbct_fp_backward_workspace_bytes(D, H, W, A, U, V)
         ws = _workspace(ws_bytes, dev)
         _check(L.cbct_fp_backward(_ptr(grad), _ptr(volume), D, H, W, *[_ptr(g) for g in geometry],
                                   A, U, V, pixel_size, step_size, voxel_size,

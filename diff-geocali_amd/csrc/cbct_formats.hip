@@ -1,0 +1,123 @@
+// Detector data -> line integrals, the input format of the projector pair (gfx950).
+//
+// The reference expects projections [A,U,V] fp32 that are already dark/flat corrected and
+// converted to line integrals, and asks callers with [A,V,U] data to transpose first
+// (README.md:97,108; docs/USAGE.md:247-251).  cbct_preprocess does both in one HBM pass:
+//   t = (raw - dark) / (flat - dark),  p = -log(max(t, t_min))     (or p = raw without log)
+// with the [A,V,U] -> [A,U,V] transpose staged through LDS so that reads run along U and
+// writes along V, both coalesced.  One workgroup = one 64x64 (V x U) tile of one view.
+#include "cbct_common.h"
+
+namespace cbct {
+namespace {
+
+constexpr int PT = 64;              // tile edge
+constexpr int PT_PAD = PT + 1;      // LDS row stride (odd: transposed reads hit distinct banks)
+
+constexpr int PV = 4;               // views per workgroup: the dark/flat terms of a pixel are loaded once
+
+// Per-pixel correction terms kept in registers across the workgroup's views:
+// p = -log(max((x - d) * r, tmin)) with r = 1 / (flat - dark), or the clamp value for a dead pixel.
+struct PixelTerms {
+    float d, r;
+    bool dead;
+};
+
+__device__ __forceinline__ PixelTerms pixel_terms(const float* dark, const float* flat, int64_t pix) {
+    PixelTerms t;
+    t.d = dark ? dark[pix] : 0.f;
+    const float den = (flat ? flat[pix] : 1.f) - t.d;
+    t.dead = !(den > 0.f);
+    t.r = t.dead ? 0.f : 1.0f / den;
+    return t;
+}
+
+__device__ __forceinline__ float line_integral(float x, const PixelTerms& t, float tmin, float log_tmin) {
+    return t.dead ? -log_tmin : -__logf(fmaxf((x - t.d) * t.r, tmin));
+}
+
+template <typename T, int TRANSPOSE>
+__global__ __launch_bounds__(256) void preprocess_kernel(const T* __restrict__ raw, const float* __restrict__ dark,
+                                                         const float* __restrict__ flat, int64_t A, int V, int U,
+                                                         float tmin, int take_log, float* __restrict__ out) {
+    __shared__ float tile[PT * PT_PAD];
+    const int tiles_u = (U + PT - 1) / PT;
+    const int tu = blockIdx.x % tiles_u, tv = blockIdx.x / tiles_u;
+    const int u0 = tu * PT, v0 = tv * PT;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;      // 64 lanes along the fast axis, 4 rows
+    const float log_tmin = __logf(tmin);
+    // this thread's 16 pixels: rows ty, ty+4, ..., fast index tx
+    PixelTerms terms[PT / 4];
+    const int f = u0 + tx;
+#pragma unroll
+    for (int k = 0; k < PT / 4; ++k) {
+        const int s = v0 + ty + 4 * k;
+        terms[k] = (take_log && s < V && f < U) ? pixel_terms(dark, flat, (int64_t)s * U + f) : PixelTerms{0.f, 1.f, false};
+    }
+    const int64_t a_end = min(A, (int64_t)(blockIdx.y + 1) * PV);
+    for (int64_t a = (int64_t)blockIdx.y * PV; a < a_end; ++a) {
+        const T* src = raw + a * V * U;
+        if (TRANSPOSE) {
+#pragma unroll
+            for (int k = 0; k < PT / 4; ++k) {
+                const int r = ty + 4 * k, s = v0 + r;
+                float p = 0.f;
+                if (s < V && f < U) {
+                    const float x = (float)src[(int64_t)s * U + f];
+                    p = take_log ? line_integral(x, terms[k], tmin, log_tmin) : x;
+                }
+                tile[r * PT_PAD + tx] = p;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PT / 4; ++k) {
+                const int r = ty + 4 * k, u = u0 + r, v = v0 + tx;
+                if (u < U && v < V) out[(a * U + u) * V + v] = tile[tx * PT_PAD + r];
+            }
+            __syncthreads();
+        } else {
+#pragma unroll
+            for (int k = 0; k < PT / 4; ++k) {
+                const int s = v0 + ty + 4 * k;
+                if (s < V && f < U) {
+                    const int64_t pix = (int64_t)s * U + f;
+                    const float x = (float)src[pix];
+                    out[a * V * U + pix] = take_log ? line_integral(x, terms[k], tmin, log_tmin) : x;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+}  // namespace cbct
+
+using namespace cbct;
+
+extern "C" int cbct_preprocess(const void* raw, int raw_dtype, const float* dark, const float* flat, int64_t A,
+                               int64_t rows, int64_t cols, int transpose, float min_transmission, int take_log,
+                               float* out, void* stream) {
+    CBCT_CHECK_ARG(raw && out, "preprocess needs input and output buffers");
+    CBCT_CHECK_ARG(A > 0 && rows > 0 && cols > 0, "projection dimensions must be positive");
+    CBCT_CHECK_ARG(A <= 65535 && rows * cols <= INT32_MAX && rows < (1 << 24) && cols < (1 << 24),
+                   "projection dimensions exceed the launch limits");
+    CBCT_CHECK_ARG(raw_dtype == 0 || raw_dtype == 1, "raw_dtype must be 0 (uint16) or 1 (float32)");
+    CBCT_CHECK_ARG(transpose == 0 || transpose == 1, "transpose must be 0 or 1");
+    CBCT_CHECK_ARG(min_transmission > 0.f && min_transmission <= 1.f, "min_transmission must be in (0, 1]");
+    const int64_t tiles = ceil_div(rows, (int64_t)PT) * ceil_div(cols, (int64_t)PT);
+    CBCT_CHECK_ARG(tiles <= INT32_MAX, "too many tiles");
+    const dim3 grid((unsigned)tiles, (unsigned)ceil_div(A, (int64_t)PV));
+    hipStream_t s = (hipStream_t)stream;
+    const int R = (int)rows, Cc = (int)cols, lg = take_log ? 1 : 0;
+    if (raw_dtype == 0) {
+        const uint16_t* r = (const uint16_t*)raw;
+        if (transpose) preprocess_kernel<uint16_t, 1><<<grid, 256, 0, s>>>(r, dark, flat, A, R, Cc, min_transmission, lg, out);
+        else preprocess_kernel<uint16_t, 0><<<grid, 256, 0, s>>>(r, dark, flat, A, R, Cc, min_transmission, lg, out);
+    } else {
+        const float* r = (const float*)raw;
+        if (transpose) preprocess_kernel<float, 1><<<grid, 256, 0, s>>>(r, dark, flat, A, R, Cc, min_transmission, lg, out);
+        else preprocess_kernel<float, 0><<<grid, 256, 0, s>>>(r, dark, flat, A, R, Cc, min_transmission, lg, out);
+    }
+    CBCT_LAUNCHED("preprocess kernel launch");
+    return CBCT_OK;
+}

@@ -213,6 +213,42 @@ struct FpTuning {
     int layout = 1;       // lane layout of a tile
 };
 static FpTuning g_tuning;
+static int g_deterministic = 0;   // volume gradient through order-independent int64 accumulation
+
+// ---- deterministic volume gradient helpers
+__global__ void zero_u64_kernel(unsigned long long* __restrict__ p, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
+}
+
+// max |x| over n floats as the bit pattern of a non-negative float (order-independent max)
+__global__ __launch_bounds__(256) void absmax_kernel(const float* __restrict__ x, int64_t n,
+                                                     unsigned* __restrict__ maxbits) {
+    float m = 0.f;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) m = fmaxf(m, fabsf(x[i]));
+    for (int s = 32; s >= 1; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(maxbits, __float_as_uint(m));
+}
+
+// global exponent E: the largest per-sample contribution |g * step| < 2^40 units of 2^-E
+__global__ void det_exponent_kernel(const unsigned* __restrict__ maxbits, float step, int* __restrict__ gexp) {
+    const float m = __uint_as_float(maxbits[0]) * step;
+    int E = 0;
+    if (m > 0.f && isfinite(m)) {
+        int k;
+        frexpf(m, &k);                // m = f * 2^k, f in [0.5, 1)
+        E = max(-60, min(120, 40 - k));
+    }
+    gexp[0] = E;
+}
+
+__global__ void det_convert_kernel(const unsigned long long* __restrict__ acc, int64_t n,
+                                   const int* __restrict__ gexp, float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = (float)ldexp((double)(long long)acc[i], -gexp[0]);
+}
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -285,6 +321,24 @@ extern "C" int cbct_fp_forward(const float* volume, int64_t D, int64_t H, int64_
     return CBCT_OK;
 }
 
+static size_t det_extra_bytes(int64_t D, int64_t H, int64_t W) {
+    return align256((size_t)D * H * W * sizeof(unsigned long long)) + 256;
+}
+
+extern "C" size_t cbct_fp_backward_workspace_bytes(int64_t D, int64_t H, int64_t W, int64_t A, int64_t U,
+                                                   int64_t V) {
+    if (D <= 0 || H <= 0 || W <= 0) return 0;
+    return cbct_fp_workspace_bytes(A, U, V) + (g_deterministic ? det_extra_bytes(D, H, W) : 0);
+}
+
+extern "C" int cbct_fp_set_deterministic(int on) {
+    CBCT_CHECK_ARG(on == 0 || on == 1, "deterministic must be 0 or 1");
+    g_deterministic = on;
+    return CBCT_OK;
+}
+
+extern "C" int cbct_fp_get_deterministic(void) { return g_deterministic; }
+
 extern "C" int cbct_fp_backward(const float* grad_projection, const float* volume,
                                 int64_t D, int64_t H, int64_t W,
                                 const float* angles, const float* sid, const float* sod,
@@ -303,23 +357,52 @@ extern "C" int cbct_fp_backward(const float* grad_projection, const float* volum
                    "grad_projection, volume and grad_geometry must be non-null");
     CBCT_CHECK_ARG(!compute_grad_volume || grad_volume != nullptr,
                    "grad_volume must be non-null when compute_grad_volume is set");
-    CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= cbct_fp_workspace_bytes(A, U, V),
-                   "workspace too small for cbct_fp_backward");
+    const bool det = g_deterministic && compute_grad_volume;
+    CBCT_CHECK_ARG(!det || g_tuning.use_slab, "the deterministic volume gradient needs the slab kernels");
+    const size_t base_bytes = cbct_fp_workspace_bytes(A, U, V);
+    CBCT_CHECK_ARG(workspace != nullptr &&
+                   workspace_bytes >= base_bytes + (det ? det_extra_bytes(D, H, W) : 0),
+                   "workspace too small for cbct_fp_backward (see cbct_fp_backward_workspace_bytes)");
     hipStream_t stream = (hipStream_t)stream_;
     ViewFrame* frames = (ViewFrame*)workspace;
     float* partials = (float*)((char*)workspace + align256((size_t)A * sizeof(ViewFrame)));
     const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
     if ((st = launch_frames(gp, A, frames, stream))) return st;
-    if (compute_grad_volume && (st = launch_fill(grad_volume, D * H * W, 0.0f, stream))) return st;
+    unsigned long long* gacc = nullptr;
+    int* gexp = nullptr;
+    const int64_t nvox = D * H * W;
+    if (det) {
+        gacc = (unsigned long long*)((char*)workspace + base_bytes);
+        unsigned* maxbits = (unsigned*)((char*)gacc + align256((size_t)nvox * sizeof(unsigned long long)));
+        gexp = (int*)(maxbits + 1);
+        const unsigned zb = (unsigned)std::min<int64_t>(ceil_div(nvox, (int64_t)256), 16384);
+        zero_u64_kernel<<<zb, 256, 0, stream>>>(gacc, nvox);
+        zero_u64_kernel<<<1, 64, 0, stream>>>((unsigned long long*)maxbits, 1);
+        CBCT_LAUNCHED("deterministic accumulator reset");
+        const int64_t ng = A * U * V;
+        absmax_kernel<<<(unsigned)std::min<int64_t>(ceil_div(ng, (int64_t)256), 8192), 256, 0, stream>>>(
+            grad_projection, ng, maxbits);
+        det_exponent_kernel<<<1, 1, 0, stream>>>(maxbits, step_size, gexp);
+        CBCT_LAUNCHED("deterministic scale kernels");
+    } else if (compute_grad_volume && (st = launch_fill(grad_volume, nvox, 0.0f, stream))) {
+        return st;
+    }
     const Box b = make_box_host(D, H, W, voxel_size, off_x, off_y, off_z);
     const int tu = (int)ceil_div(U, FP_TILE), tv = (int)ceil_div(V, FP_TILE);
     const int nblocks = (int)(A * tu * tv);
     const float inv_vs = 1.0f / voxel_size;
     if (g_tuning.use_slab) {
-        const SlabArgs args{volume, grad_projection, frames, nullptr, compute_grad_volume ? grad_volume : nullptr,
-                            partials, b, (int)U, (int)V, tu, tv, nblocks, pixel_size, step_size, voxel_size,
-                            inv_vs, g_tuning.T, g_tuning.cap, g_tuning.layout};
+        SlabArgs args{volume, grad_projection, frames, nullptr, compute_grad_volume ? grad_volume : nullptr,
+                      partials, b, (int)U, (int)V, tu, tv, nblocks, pixel_size, step_size, voxel_size,
+                      inv_vs, g_tuning.T, g_tuning.cap, g_tuning.layout};
+        args.gacc = gacc;
+        args.gexp = gexp;
         if ((st = launch_fp_slab(compute_grad_volume ? 2 : 1, false, args, stream))) return st;
+        if (det) {
+            det_convert_kernel<<<(unsigned)ceil_div(nvox, (int64_t)256), 256, 0, stream>>>(gacc, nvox, gexp,
+                                                                                         grad_volume);
+            CBCT_LAUNCHED("deterministic volume gradient conversion");
+        }
     } else if (compute_grad_volume)
         fp_backward_kernel<true><<<nblocks, FP_THREADS, 0, stream>>>(
             grad_projection, volume, b, frames, (int)U, (int)V, tu, tv, nblocks, pixel_size,

@@ -22,6 +22,11 @@ struct SlabArgs {
     int cap;      // LDS floats per box
     int layout;   // 0: each wave owns an 8x8 pixel block; 1: waves interleave with stride 2
     float resid_scale;   // mode 3: dL/dproj = resid_scale * (proj - measured)
+    // Deterministic volume gradient (mode 2): when gacc is set, contributions are added as
+    // integers in units of 2^-(*gexp) to gacc (int64 two's complement), so the sum does not
+    // depend on the order of the atomics; a later kernel converts gacc to gvol.
+    unsigned long long* gacc;
+    const int* gexp;
 };
 
 // One calibration problem of a batched launch (device resident).

@@ -86,6 +86,9 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     // a voxel's 2x2x2 support (floor(2 sqrt3 vs / step) + 1) and n_rays bounds the tile's rays
     // through that support (pixels within its projected radius, or all 256).
     float gsq = 0.f, inv_scale = 1.f;
+    int e_tile = 0;                                   // the tile's fixed-point exponent
+    const int E = (MODE == 2 && P.gacc) ? *P.gexp : 0;  // the global exponent (deterministic mode)
+    const double scaleE = ldexp(1.0, E);
     __shared__ float red_max[SL_THREADS / 64];
     unsigned* gboxu = reinterpret_cast<unsigned*>(gbox);
     if (MODE == 2) {
@@ -127,6 +130,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         }
         gsq = gs * ldexpf(1.f, e);
         inv_scale = ldexpf(1.f, -e);
+        e_tile = e;
     }
     float gox = 0.f, goy = 0.f, goz = 0.f, gdx = 0.f, gdy = 0.f, gdz = 0.f, gtmin = 0.f;
 
@@ -161,7 +165,22 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         const int i000 = (c.z0 * b.H + c.y0) * b.W + c.x0;
         const int ox1 = c.sx, oy1 = c.sy * b.W, oz1 = c.sz * plane;
         const Taps tp = load_taps_strided(P.vol, i000, ox1, oy1, oz1);
-        if (MODE == 2) {
+        if (MODE == 2 && P.gacc) {
+            const Weights w = tri_weights(c.fx, c.fy, c.fz);
+            unsigned long long* q = P.gacc + i000;
+            auto add = [&](unsigned long long* dst, float v) {
+                const long long iv = (long long)rint((double)v * scaleE);
+                if (iv != 0) atomicAdd(dst, (unsigned long long)iv);
+            };
+            add(q, gs * w.w000);
+            add(q + ox1, gs * w.w100);
+            add(q + oy1, gs * w.w010);
+            add(q + oy1 + ox1, gs * w.w110);
+            add(q + oz1, gs * w.w001);
+            add(q + oz1 + ox1, gs * w.w101);
+            add(q + oz1 + oy1, gs * w.w011);
+            add(q + oz1 + oy1 + ox1, gs * w.w111);
+        } else if (MODE == 2) {
             const Weights w = tri_weights(c.fx, c.fy, c.fz);
             float* q = P.gvol + i000;
             atomicAdd(q, gs * w.w000);
@@ -333,7 +352,14 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                         const int z = box_index_div((unsigned)r, (unsigned)B1, m1);
                         const int y = min(max(lo[1] + (r - z * B1), 0), b.H - 1);
                         const int zz = min(max(lo[2] + z, 0), b.D - 1);
-                        atomicAdd(P.gvol + (zz * b.H + y) * b.W + x, gval);
+                        if (P.gacc) {
+                            const long long q = (long long)ival;
+                            const int sh = E - e_tile;
+                            const long long iv = sh >= 0 ? (q << min(sh, 62)) : (q >> min(-sh, 62));
+                            if (iv != 0) atomicAdd(P.gacc + (zz * b.H + y) * b.W + x, (unsigned long long)iv);
+                        } else {
+                            atomicAdd(P.gvol + (zz * b.H + y) * b.W + x, gval);
+                        }
                     }
                 }
             }

@@ -56,6 +56,19 @@ int cbct_fp_forward(const float* volume, int64_t D, int64_t H, int64_t W,
                     float off_x, float off_y, float off_z,
                     float* projections, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Workspace of cbct_fp_backward: cbct_fp_workspace_bytes plus, in deterministic
+ * mode, 8 bytes per voxel (+256) for the integer accumulator.  Query it with the
+ * deterministic setting that the call will use. */
+size_t cbct_fp_backward_workspace_bytes(int64_t D, int64_t H, int64_t W, int64_t A, int64_t U, int64_t V);
+
+/* Deterministic mode (process-wide, default off): the projector adjoint's volume
+ * gradient is accumulated as int64 fixed-point values (scale 2^-E with E chosen
+ * from max|grad * step|, so each sample keeps >= 40 bits), which makes the result
+ * bitwise reproducible run to run; the reference's float atomics are not
+ * (docs/USAGE.md:306).  Needs the slab kernels (use_slab = 1). */
+int cbct_fp_set_deterministic(int on);
+int cbct_fp_get_deterministic(void);
+
 /* Adjoint of cbct_fp_forward for the upstream gradient grad_projection[A,U,V]:
  * grad_volume[D,H,W] (skipped and may be NULL when compute_grad_volume == 0)
  * and grad_geometry[8][A].
@@ -201,6 +214,19 @@ int cbct_calib_run(const cbct_calib_problem* problems, int nproblems, int nsteps
 size_t cbct_calib_lm_workspace_bytes(const cbct_calib_problem* problems, int nproblems);
 int cbct_calib_run_lm(const cbct_calib_problem* problems, int nproblems, int nsteps, float lambda0,
                       int use_graph, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Detector data -> line integrals [A][U][V] fp32 (the projector's input format).
+ * raw: [A][rows][cols] of uint16 (raw_dtype 0) or float32 (1).  transpose = 1:
+ * raw is [A][V][U] (rows = V, cols = U) and is transposed to [A][U][V];
+ * transpose = 0: raw is already [A][U][V] (rows = U, cols = V).  With take_log,
+ * t = (raw - dark) / (flat - dark) per pixel (dark, flat: [rows][cols] fp32 in the
+ * raw layout; NULL = 0 and 1) and out = -log(max(t, min_transmission)); a pixel
+ * with flat <= dark gives -log(min_transmission).  Without take_log, out = raw
+ * (conversion and transpose only).  Replaces the preprocessing the reference
+ * leaves to the caller (README.md:97,108; docs/USAGE.md:247-251). */
+int cbct_preprocess(const void* raw, int raw_dtype, const float* dark, const float* flat, int64_t A,
+                    int64_t rows, int64_t cols, int transpose, float min_transmission, int take_log,
+                    float* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -503,3 +503,18 @@ def point_detector_coordinates(points, geometry, pixel_size):
         vi[a] = X[:, 2] / pixel_size + v0[a]
         w[a] = ((S @ en) / ((S - P) @ en)) ** 2
     return ui, vi, w
+
+
+def preprocess(raw, dark=None, flat=None, layout="AVU", min_transmission=1e-6, log=True):
+    """FP64 restatement of the detector preprocessing: [A,V,U] (or [A,U,V]) raw frames ->
+    [A,U,V] line integrals -log(max((raw - dark) / (flat - dark), t_min)); flat <= dark
+    gives -log(t_min).  The caller's step in the reference (README.md:97,108)."""
+    x = np.asarray(raw, dtype=np.float64)
+    if log:
+        d = np.zeros(x.shape[1:]) if dark is None else np.asarray(dark, np.float64)
+        f = np.ones(x.shape[1:]) if flat is None else np.asarray(flat, np.float64)
+        den = f - d
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t = np.where(den > 0, (x - d) / np.where(den > 0, den, 1.0), min_transmission)
+        x = -np.log(np.maximum(t, min_transmission))
+    return np.ascontiguousarray(x.transpose(0, 2, 1)) if layout == "AVU" else x

@@ -36,6 +36,12 @@ def test_version_and_workspace_queries():
     L = _native.lib()
     assert L.cbct_fp_workspace_bytes(720, 768, 640) > 720 * 64
     assert L.cbct_fp_workspace_bytes(0, 4, 4) == 0
+    base = L.cbct_fp_backward_workspace_bytes(64, 64, 64, 10, 32, 32)
+    assert base == L.cbct_fp_workspace_bytes(10, 32, 32)
+    with _native.deterministic():
+        det = L.cbct_fp_backward_workspace_bytes(64, 64, 64, 10, 32, 32)
+    assert det == base + 64 ** 3 * 8 + 256
+    assert not _native.get_deterministic()
     assert L.cbct_bp_workspace_bytes(360, 256, 256, 256) == 360 * 64
     assert L.cbct_fdk_workspace_bytes(360, 512, 384) >= 360 * 512 * 384 * 4
 

@@ -67,3 +67,32 @@ def test_calib_entry_point_rejects_bad_arguments_without_a_gpu():
     st = lib.cbct_calib_run(ctypes.addressof(prob), 1, 3, 0, ctypes.addressof(lr), 1.0, 0.9, 0.999, 1e-8, 0,
                             None, 0, None)
     assert st != 0 and b"workspace" in lib.cbct_last_error()
+
+
+def test_geometry_from_table_follows_the_csv_convention(tmp_path):
+    from cbct_backprojector import formats
+    from cbct_backprojector.operator import detector_centers
+    A, U, V, pix = 4, 100, 80, 0.5
+    rows = ["SAD_Rs,DD,delta_U,delta_V,delta_src_x,delta_src_y,delta_src_z,delta_Beta"]
+    for k in range(A):
+        rows.append(f"{600 + k},{1100 - k},{0.1 * k},{-0.2 * k},{0.01 * k},{0.02 * k},{0.03 * k},{0.001 * k}")
+    path = tmp_path / "geom.csv"
+    path.write_text("\n".join(rows) + "\n")
+    g = formats.geometry_from_table(str(path), U, V, pix)
+    k = torch.arange(A, dtype=torch.float64)
+    assert torch.allclose(g["sod"].double(), 600 + k) and torch.allclose(g["sid"].double(), 1100 - k)
+    u0, v0 = detector_centers(0.1 * k, -0.2 * k, U, V, pix)
+    assert torch.allclose(g["det_u0"].double(), u0.double(), atol=1e-5)
+    assert torch.allclose(g["det_v0"].double(), v0.double(), atol=1e-5)
+    assert torch.allclose(g["angles"].double(), 2 * math.pi * k / A + 0.001 * k, atol=1e-6)
+    assert torch.allclose(g["src_z"].double(), 0.03 * k, atol=1e-7)
+    assert list(g) == ["angles", "sid", "sod", "det_u0", "det_v0", "src_x", "src_y", "src_z"]
+
+
+def test_geometry_from_table_rejects_missing_or_ragged_columns():
+    import pytest
+    from cbct_backprojector import formats
+    with pytest.raises(ValueError, match="SAD_Rs"):
+        formats.geometry_from_table({"DD": [1.0]}, 4, 4, 1.0)
+    with pytest.raises(ValueError, match="one value per view"):
+        formats.geometry_from_table({"SAD_Rs": [1.0, 2.0], "DD": [3.0]}, 4, 4, 1.0)

@@ -331,3 +331,25 @@ def test_fdk_chunk_invariance():
     for c in (1, 3, 7):
         np.testing.assert_allclose(oracle.reconstruct_fdk(proj, geom, 1.0, (6, 8, 8), 1.0, angle_chunk_size=c),
                                    ref, rtol=1e-12, atol=1e-12)
+
+
+def test_preprocess_oracle_recovers_line_integrals_and_transposes():
+    """Known answer: frames made from line integrals p by Beer-Lambert with dark and flat
+    fields give back p (transposed from [A,V,U] to [A,U,V]); fully absorbed or dead pixels
+    clamp at -log(t_min)."""
+    rng = np.random.default_rng(3)
+    A, V, U = 2, 5, 7
+    p = rng.uniform(0.0, 4.0, (A, U, V))
+    dark = rng.uniform(90, 110, (V, U))
+    flat = rng.uniform(3000, 4000, (V, U))
+    raw = dark + (flat - dark) * np.exp(-p.transpose(0, 2, 1))
+    got = oracle.preprocess(raw, dark, flat, "AVU")
+    assert got.shape == (A, U, V)
+    assert np.allclose(got, p, rtol=1e-12, atol=1e-12)
+    raw2 = raw.copy()
+    raw2[0, 0, 0] = dark[0, 0] - 5            # below dark: transmission clamps
+    flat2 = flat.copy()
+    flat2[1, 2] = dark[1, 2]                    # dead pixel
+    got2 = oracle.preprocess(raw2, dark, flat2, "AVU", min_transmission=1e-4)
+    assert got2[0, 0, 0] == -np.log(1e-4) and got2[0, 2, 1] == -np.log(1e-4)
+    assert np.array_equal(oracle.preprocess(raw, layout="AUV", log=False), raw)

@@ -52,6 +52,9 @@ _SIGNATURES = {
     "cbct_fdk_filter": ([_P, _I64, _I64, _I64, _F, _P, _P, _F, _P, _P, _SZ, _P], _INT),
     "cbct_fdk_reconstruct": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
                              _F, _I64, _P, _P, _SZ, _P], _INT),
+    "cbct_fdk_filter_ex": ([_P, _I64, _I64, _I64, _F, _P, _P, _F, _P, _P, _P, _P, _P, _SZ, _P], _INT),
+    "cbct_fdk_reconstruct_ex": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
+                                _F, _I64, _INT, _INT, _P, _P, _SZ, _P], _INT),
     "cbct_calib_workspace_bytes": ([_P, _INT], _SZ),
     "cbct_calib_run": ([_P, _INT, _INT, _INT, _P, _F, _F, _F, _F, _INT, _P, _SZ, _P], _INT),
     "cbct_calib_lm_workspace_bytes": ([_P, _INT], _SZ),
@@ -263,23 +266,25 @@ def bp_backward(grad_vol, proj, geometry, pixel_size, voxel_size, off_x, off_y, 
     return gproj, ggeo
 
 
-def fdk_filter(proj, pixel_size, sod, sid, alpha):
-    """sod, sid: contiguous float32 [A] tensors on the projections' device."""
+def fdk_filter(proj, pixel_size, sod, sid, alpha, det_u0=None, det_v0=None, angles=None):
+    """sod, sid: contiguous float32 [A] tensors on the projections' device; det_u0/det_v0
+    (both or neither): per-view preweight centres; angles: Parker short-scan weights."""
     _require_gpu(proj, "projections")
     L = lib()
     A, U, V = proj.shape
     dev = proj.device
+    opt = lambda t: None if t is None else _ptr(t)
     with torch.cuda.device(dev):
         out = torch.empty((A, U, V), dtype=torch.float32, device=dev)
         ws_bytes = L.cbct_fdk_workspace_bytes(A, U, V)
         ws = _workspace(ws_bytes, dev)
-        _check(L.cbct_fdk_filter(_ptr(proj), A, U, V, pixel_size, _ptr(sod), _ptr(sid), alpha,
-                                 _ptr(out), _ptr(ws), ws_bytes, _stream(dev)))
+        _check(L.cbct_fdk_filter_ex(_ptr(proj), A, U, V, pixel_size, _ptr(sod), _ptr(sid), alpha, opt(det_u0),
+                                    opt(det_v0), opt(angles), _ptr(out), _ptr(ws), ws_bytes, _stream(dev)))
     return out
 
 
 def fdk_reconstruct(proj, geometry, pixel_size, D, H, W, voxel_size, off_x, off_y, off_z, alpha,
-                    angle_chunk_size):
+                    angle_chunk_size, short_scan=False, offset_preweight=False):
     _require_gpu(proj, "projections")
     L = lib()
     A, U, V = proj.shape
@@ -288,7 +293,8 @@ def fdk_reconstruct(proj, geometry, pixel_size, D, H, W, voxel_size, off_x, off_
         vol = torch.empty((D, H, W), dtype=torch.float32, device=dev)
         ws_bytes = L.cbct_fdk_workspace_bytes(A, U, V)
         ws = _workspace(ws_bytes, dev)
-        _check(L.cbct_fdk_reconstruct(_ptr(proj), A, U, V, *[_ptr(g) for g in geometry], pixel_size,
-                                      D, H, W, voxel_size, off_x, off_y, off_z, alpha,
-                                      angle_chunk_size, _ptr(vol), _ptr(ws), ws_bytes, _stream(dev)))
+        _check(L.cbct_fdk_reconstruct_ex(_ptr(proj), A, U, V, *[_ptr(g) for g in geometry], pixel_size,
+                                         D, H, W, voxel_size, off_x, off_y, off_z, alpha, angle_chunk_size,
+                                         1 if short_scan else 0, 1 if offset_preweight else 0,
+                                         _ptr(vol), _ptr(ws), ws_bytes, _stream(dev)))
     return vol

@@ -9,6 +9,13 @@ convolution restated exactly as a direct linear convolution along U.
 As in the reference, the cosine preweight uses centred detector coordinates
 without detector/source offsets; this is the project's FDK approximation for
 complete, uniformly sampled circular scans.  The filter runs on the GPU only.
+
+Two keyword-only extensions go beyond what the reference supports
+(filtering.py:1-8 and docs/USAGE.md:332 leave them to the user):
+``short_scan=True`` reconstructs a scan over pi plus the fan angle with Parker
+weights (angles increasing, range > pi; weight of a view = the mean angular
+step instead of pi/A), and ``offset_preweight=True`` centres the cosine
+preweight on each view's det_u0/det_v0 instead of the detector's middle.
 """
 
 import math
@@ -55,7 +62,7 @@ def _filter_distance(name, value, projections):
     return distance.contiguous()
 
 
-def fdk_filter(projections, pixel_size, sod, sid, alpha=0.8):
+def fdk_filter(projections, pixel_size, sod, sid, alpha=0.8, *, det_u0=None, det_v0=None, angles=None):
     """Apply centred cosine weighting and a Hann-windowed discrete ramp filter (filtering.py:54-104).
 
     Args:
@@ -70,6 +77,10 @@ def fdk_filter(projections, pixel_size, sod, sid, alpha=0.8):
         filtered projections, equal to the reference's filtering along U with
         zero padding to the next power of two at least 2U, followed by the
         ``pixel_size * sid / sod`` scale factor.
+
+    Keyword-only (not in the reference): ``det_u0``/``det_v0`` ([A] tensors, both or
+    neither) centre the preweight per view; ``angles`` ([A], increasing, range > pi)
+    applies Parker short-scan weights before the ramp filter.
     """
     _check_projections(projections)
     pixel_size = _positive_scalar("pixel_size", pixel_size)
@@ -78,7 +89,32 @@ def fdk_filter(projections, pixel_size, sod, sid, alpha=0.8):
     sid = _filter_distance("sid", sid, projections)
     if projections.device.type != "cuda":
         raise ValueError("fdk_filter runs on the GPU in this build; move projections to a GPU device")
-    return _native.fdk_filter(projections.contiguous(), pixel_size, sod, sid, alpha)
+    if (det_u0 is None) != (det_v0 is None):
+        raise ValueError("det_u0 and det_v0 must be given together")
+    extra = []
+    for name, values in (("det_u0", det_u0), ("det_v0", det_v0), ("angles", angles)):
+        if values is not None:
+            values = _per_view(name, values, projections)
+        extra.append(values)
+    if angles is not None:
+        _check_short_scan(extra[2])
+    return _native.fdk_filter(projections.contiguous(), pixel_size, sod, sid, alpha, *extra)
+
+
+def _per_view(name, values, projections):
+    values = torch.as_tensor(values, dtype=projections.dtype, device=projections.device).reshape(-1)
+    if values.numel() != projections.shape[0]:
+        raise ValueError(f"{name} must have one value per view")
+    if not bool(torch.isfinite(values).all()):
+        raise ValueError(f"{name} must contain only finite values")
+    return values.contiguous()
+
+
+def _check_short_scan(angles):
+    if angles.numel() < 2 or not bool((angles[1:] > angles[:-1]).all()):
+        raise ValueError("a short-scan reconstruction needs strictly increasing angles")
+    if float(angles[-1] - angles[0]) <= math.pi:
+        raise ValueError("a short-scan reconstruction needs angles spanning more than pi")
 
 
 @torch.no_grad()
@@ -98,6 +134,9 @@ def reconstruct_fdk(
     volume_offset=(0.0, 0.0, 0.0),
     alpha=0.8,
     angle_chunk_size=32,
+    *,
+    short_scan=False,
+    offset_preweight=False,
 ):
     """Filter and backproject in view chunks, then apply ``pi / A`` and clamp (filtering.py:107-198).
 
@@ -108,6 +147,10 @@ def reconstruct_fdk(
 
     The chunks are summed exactly as the reference orders them (each chunk from
     zero, then added to the running volume), inside one fused kernel.
+
+    Keyword-only (not in the reference): ``short_scan`` (Parker weights; the angles must
+    increase and span more than pi; each view is weighted by the mean angular step) and
+    ``offset_preweight`` (preweight centred on det_u0/det_v0).
     """
     _check_projections(projections)
     if not projections.is_cuda:
@@ -151,6 +194,9 @@ def reconstruct_fdk(
     if len(volume_offset) != 3 or not all(math.isfinite(value) for value in volume_offset):
         raise ValueError("volume_offset must contain three finite values")
 
+    if short_scan:
+        _check_short_scan(angles)
     tensors = tuple(values.contiguous() for values in geometry)
     return _native.fdk_reconstruct(projections.contiguous(), tensors, pixel_size, *volume_shape,
-                                   voxel_size, *volume_offset, alpha, angle_chunk_size)
+                                   voxel_size, *volume_offset, alpha, angle_chunk_size,
+                                   bool(short_scan), bool(offset_preweight))

@@ -84,6 +84,7 @@ constexpr int FT_U = 64, FT_V = 64;
 __global__ __launch_bounds__(256) void fdk_filter_kernel(
     const float* __restrict__ proj, int A, int U, int V, float pix,
     const float* __restrict__ sod, const float* __restrict__ sid,
+    const float* __restrict__ u0a, const float* __restrict__ v0a, const float* __restrict__ pw,
     const float* __restrict__ taps_g, float* __restrict__ out) {
     extern __shared__ float smem[];
     const int ntaps = 2 * U - 1 + 2 * TAP_PAD;
@@ -98,7 +99,9 @@ __global__ __launch_bounds__(256) void fdk_filter_kernel(
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < ntaps; i += 256) taps[i] = taps_g[i];
     const float sid_a = sid[a], sod_a = sod[a];
-    const float cu = (float)(U - 1) * 0.5f, cv = (float)(V - 1) * 0.5f;
+    // detector centre of the cosine preweight: the geometric centre (the reference's
+    // convention), or this view's det_u0/det_v0 when given
+    const float cu = u0a ? u0a[a] : (float)(U - 1) * 0.5f, cv = v0a ? v0a[a] : (float)(V - 1) * 0.5f;
     const float sid2 = sid_a * sid_a;
     const int v = vb + lane;
     float acc[16];
@@ -118,6 +121,7 @@ __global__ __launch_bounds__(256) void fdk_filter_kernel(
                 const float vp = ((float)gv - cv) * pix;
                 const float dist = sqrtf((sid2 + up * up) + vp * vp);
                 x = pa[(int64_t)gu * V + gv] * (sid_a / dist);
+                if (pw) x *= pw[(int64_t)a * U + gu];
             }
             X[uu * (FT_V + 1) + vv] = x;
         }
@@ -147,10 +151,43 @@ __global__ __launch_bounds__(256) void fdk_filter_kernel(
     }
 }
 
+// Parker short-scan weights per (view, detector column) (Parker 1982; Kak & Slaney ch. 3):
+// with beta = angle - angle_0, fan angle gamma = atan(u_p / sid) (u_p along the detector u
+// axis from the preweight centre) and delta = (range - pi) / 2 for the scan range
+// angle_{A-1} - angle_0, the ray (beta, gamma) is measured again as (beta + pi + 2 gamma, -gamma)
+// in this geometry, and
+//   w = sin^2(pi/4 * beta / (delta - gamma))              for 0 <= beta < 2 delta - 2 gamma
+//     = 1                                                  up to pi - 2 gamma
+//     = sin^2(pi/4 * (pi + 2 delta - beta) / (delta + gamma)) up to pi + 2 delta
+// makes every line's weights sum to one.
+__global__ void parker_weights_kernel(const float* __restrict__ angles, int A, int U,
+                                      const float* __restrict__ u0a, float pix,
+                                      const float* __restrict__ sid, float* __restrict__ w) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)A * U) return;
+    const int a = (int)(i / U), u = (int)(i % U);
+    const float pi = 3.14159265358979f;
+    const float beta = angles[a] - angles[0];
+    const float delta = fmaxf((angles[A - 1] - angles[0] - pi) * 0.5f, 1e-9f);
+    const float cu = u0a ? u0a[a] : (float)(U - 1) * 0.5f;
+    const float g = atanf(((float)u - cu) * pix / sid[a]);
+    float val = 0.f;
+    if (beta >= 0.f && beta < 2.f * delta - 2.f * g) {
+        const float t = sinf(pi / 4.f * beta / (delta - g));
+        val = t * t;
+    } else if (beta >= 2.f * delta - 2.f * g && beta <= pi - 2.f * g) {
+        val = 1.f;
+    } else if (beta > pi - 2.f * g && beta <= pi + 2.f * delta) {
+        const float t = sinf(pi / 4.f * (pi + 2.f * delta - beta) / (delta + g));
+        val = t * t;
+    }
+    w[i] = val;
+}
+
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct FdkLayout {
-    size_t frames, rh, taps, filtered, total;
+    size_t frames, rh, taps, parker, filtered, total;
 };
 
 static FdkLayout fdk_layout(int64_t A, int64_t U, int64_t V) {
@@ -158,24 +195,32 @@ static FdkLayout fdk_layout(int64_t A, int64_t U, int64_t V) {
     L.frames = 0;
     L.rh = align256((size_t)A * sizeof(ViewFrame));
     L.taps = L.rh + align256((size_t)fdk_pad(U) * sizeof(float));
-    L.filtered = L.taps + align256((size_t)(2 * U - 1 + 2 * TAP_PAD) * sizeof(float));
+    L.parker = L.taps + align256((size_t)(2 * U - 1 + 2 * TAP_PAD) * sizeof(float));
+    L.filtered = L.parker + align256((size_t)A * U * sizeof(float));
     L.total = L.filtered + (size_t)A * U * V * sizeof(float);
     return L;
 }
 
 static int launch_filter(const float* proj, int64_t A, int64_t U, int64_t V, float pix,
-                         const float* sod, const float* sid, float alpha, float* out,
-                         float* rh, float* taps, hipStream_t stream) {
+                         const float* sod, const float* sid, float alpha, const float* u0a, const float* v0a,
+                         const float* angles, float* out, float* rh, float* taps, float* pw,
+                         hipStream_t stream) {
     const int pad = (int)fdk_pad(U);
     ramp_response_kernel<<<(pad + 255) / 256, 256, 0, stream>>>(pad, pix, alpha, rh);
     CBCT_LAUNCHED("ramp response kernel launch");
     const int ntaps = (int)(2 * U - 1 + 2 * TAP_PAD);
     taps_kernel<<<(ntaps + 255) / 256, 256, 0, stream>>>(rh, pad, (int)U, taps);
     CBCT_LAUNCHED("filter taps kernel launch");
+    if (angles) {
+        parker_weights_kernel<<<(unsigned)ceil_div(A * U, (int64_t)256), 256, 0, stream>>>(angles, (int)A, (int)U,
+                                                                                          u0a, pix, sid, pw);
+        CBCT_LAUNCHED("Parker weights kernel launch");
+    }
     const int64_t blocks = A * ceil_div(U, FT_U) * ceil_div(V, FT_V);
     const size_t shmem = (size_t)(((ntaps + 3) & ~3) + FT_U * (FT_V + 1)) * sizeof(float);
     fdk_filter_kernel<<<(unsigned)blocks, 256, shmem, stream>>>(proj, (int)A, (int)U, (int)V, pix,
-                                                                sod, sid, taps, out);
+                                                                sod, sid, u0a, v0a, angles ? pw : nullptr, taps,
+                                                                out);
     CBCT_LAUNCHED("FDK filter kernel launch");
     return CBCT_OK;
 }
@@ -200,17 +245,27 @@ extern "C" size_t cbct_fdk_workspace_bytes(int64_t A, int64_t U, int64_t V) {
     return fdk_layout(A, U, V).total;
 }
 
-extern "C" int cbct_fdk_filter(const float* projections, int64_t A, int64_t U, int64_t V,
-                               float pixel_size, const float* sod, const float* sid, float alpha,
-                               float* filtered, void* workspace, size_t workspace_bytes, void* stream_) {
+extern "C" int cbct_fdk_filter_ex(const float* projections, int64_t A, int64_t U, int64_t V,
+                                  float pixel_size, const float* sod, const float* sid, float alpha,
+                                  const float* det_u0, const float* det_v0, const float* angles,
+                                  float* filtered, void* workspace, size_t workspace_bytes, void* stream_) {
     int st;
     if ((st = check_fdk_common(A, U, V, pixel_size, alpha))) return st;
     CBCT_CHECK_ARG(projections && sod && sid && filtered, "projection, distance and output pointers must be non-null");
+    CBCT_CHECK_ARG((det_u0 == nullptr) == (det_v0 == nullptr), "det_u0 and det_v0 must both be given or both be null");
+    CBCT_CHECK_ARG(!angles || A >= 2, "a short-scan (Parker) filter needs at least two views");
     const FdkLayout L = fdk_layout(A, U, V);
     CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= L.filtered, "workspace too small for cbct_fdk_filter");
     char* ws = (char*)workspace;
-    return launch_filter(projections, A, U, V, pixel_size, sod, sid, alpha, filtered,
-                         (float*)(ws + L.rh), (float*)(ws + L.taps), (hipStream_t)stream_);
+    return launch_filter(projections, A, U, V, pixel_size, sod, sid, alpha, det_u0, det_v0, angles, filtered,
+                         (float*)(ws + L.rh), (float*)(ws + L.taps), (float*)(ws + L.parker), (hipStream_t)stream_);
+}
+
+extern "C" int cbct_fdk_filter(const float* projections, int64_t A, int64_t U, int64_t V,
+                               float pixel_size, const float* sod, const float* sid, float alpha,
+                               float* filtered, void* workspace, size_t workspace_bytes, void* stream_) {
+    return cbct_fdk_filter_ex(projections, A, U, V, pixel_size, sod, sid, alpha, nullptr, nullptr, nullptr,
+                              filtered, workspace, workspace_bytes, stream_);
 }
 
 namespace cbct {
@@ -219,14 +274,14 @@ int launch_bp_forward_fdk(const float* proj, const ViewFrame* frames, int64_t A,
                           float oz, int chunk, float scale, float* vol, hipStream_t stream);
 }
 
-extern "C" int cbct_fdk_reconstruct(const float* projections, int64_t A, int64_t U, int64_t V,
-                                    const float* angles, const float* sid, const float* sod,
-                                    const float* det_u0, const float* det_v0,
-                                    const float* src_x, const float* src_y, const float* src_z,
-                                    float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
-                                    float off_x, float off_y, float off_z,
-                                    float alpha, int64_t angle_chunk_size,
-                                    float* volume, void* workspace, size_t workspace_bytes, void* stream_) {
+extern "C" int cbct_fdk_reconstruct_ex(const float* projections, int64_t A, int64_t U, int64_t V,
+                                       const float* angles, const float* sid, const float* sod,
+                                       const float* det_u0, const float* det_v0,
+                                       const float* src_x, const float* src_y, const float* src_z,
+                                       float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
+                                       float off_x, float off_y, float off_z,
+                                       float alpha, int64_t angle_chunk_size, int short_scan, int offset_preweight,
+                                       float* volume, void* workspace, size_t workspace_bytes, void* stream_) {
     int st;
     if ((st = check_fdk_common(A, U, V, pixel_size, alpha))) return st;
     CBCT_CHECK_ARG(U >= 2 && V >= 2, "reconstruct_fdk requires detector dimensions U and V >= 2");
@@ -236,6 +291,7 @@ extern "C" int cbct_fdk_reconstruct(const float* projections, int64_t A, int64_t
     CBCT_CHECK_ARG(angle_chunk_size >= 1, "angle_chunk_size must be at least 1");
     CBCT_CHECK_ARG(D > 0 && H > 0 && W > 0 && D * H * W <= INT32_MAX,
                    "volume_shape must be positive and within the 2^31-1 element limit");
+    CBCT_CHECK_ARG(!short_scan || A >= 2, "a short-scan reconstruction needs at least two views");
     if ((st = check_positive_finite(voxel_size, "voxel_size"))) return st;
     if ((st = check_finite(off_x, "off_x")) || (st = check_finite(off_y, "off_y")) ||
         (st = check_finite(off_z, "off_z")))
@@ -246,13 +302,39 @@ extern "C" int cbct_fdk_reconstruct(const float* projections, int64_t A, int64_t
     char* ws = (char*)workspace;
     ViewFrame* frames = (ViewFrame*)(ws + L.frames);
     float* filtered = (float*)(ws + L.filtered);
+    double scale = 3.14159265358979323846 / (double)A;       // full scan: (1/2) * 2 pi / A
+    if (short_scan) {
+        // Parker-weighted short scan: weight = the mean angular step (reads the angular range)
+        float ends[2];
+        CBCT_HIP(hipMemcpyAsync(&ends[0], angles, sizeof(float), hipMemcpyDeviceToHost, stream), "angle read");
+        CBCT_HIP(hipMemcpyAsync(&ends[1], angles + (A - 1), sizeof(float), hipMemcpyDeviceToHost, stream),
+                 "angle read");
+        CBCT_HIP(hipStreamSynchronize(stream), "angle read");
+        const double range = (double)ends[1] - (double)ends[0];
+        CBCT_CHECK_ARG(range > 3.14159265358979323846,
+                       "a short-scan reconstruction needs increasing angles spanning more than pi");
+        scale = range / (double)(A - 1);
+    }
     const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
     if ((st = launch_frames(gp, A, frames, stream))) return st;
-    if ((st = launch_filter(projections, A, U, V, pixel_size, sod, sid, alpha, filtered,
-                            (float*)(ws + L.rh), (float*)(ws + L.taps), stream)))
+    if ((st = launch_filter(projections, A, U, V, pixel_size, sod, sid, alpha, offset_preweight ? det_u0 : nullptr,
+                            offset_preweight ? det_v0 : nullptr, short_scan ? angles : nullptr, filtered,
+                            (float*)(ws + L.rh), (float*)(ws + L.taps), (float*)(ws + L.parker), stream)))
         return st;
     const int chunk = (int)std::min<int64_t>(angle_chunk_size, A);
-    const float scale = (float)(3.14159265358979323846 / (double)A);
     return launch_bp_forward_fdk(filtered, frames, A, U, V, D, H, W, pixel_size, voxel_size,
-                                 off_x, off_y, off_z, chunk, scale, volume, stream);
+                                 off_x, off_y, off_z, chunk, (float)scale, volume, stream);
+}
+
+extern "C" int cbct_fdk_reconstruct(const float* projections, int64_t A, int64_t U, int64_t V,
+                                    const float* angles, const float* sid, const float* sod,
+                                    const float* det_u0, const float* det_v0,
+                                    const float* src_x, const float* src_y, const float* src_z,
+                                    float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
+                                    float off_x, float off_y, float off_z,
+                                    float alpha, int64_t angle_chunk_size,
+                                    float* volume, void* workspace, size_t workspace_bytes, void* stream_) {
+    return cbct_fdk_reconstruct_ex(projections, A, U, V, angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z,
+                                   pixel_size, D, H, W, voxel_size, off_x, off_y, off_z, alpha, angle_chunk_size,
+                                   0, 0, volume, workspace, workspace_bytes, stream_);
 }

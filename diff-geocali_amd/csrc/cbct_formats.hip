@@ -14,7 +14,7 @@ namespace {
 constexpr int PT = 64;              // tile edge
 constexpr int PT_PAD = PT + 1;      // LDS row stride (odd: transposed reads hit distinct banks)
 
-constexpr int PV = 4;               // views per workgroup: the dark/flat terms of a pixel are loaded once
+constexpr int PV = 1;               // views per workgroup (more views reuse the dark/flat terms but cost parallelism)
 
 // Per-pixel correction terms kept in registers across the workgroup's views:
 // p = -log(max((x - d) * r, tmin)) with r = 1 / (flat - dark), or the clamp value for a dead pixel.

@@ -163,6 +163,28 @@ int cbct_fdk_reconstruct(const float* projections, int64_t A, int64_t U, int64_t
                          float alpha, int64_t angle_chunk_size,
                          float* volume, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Extensions of the FDK helpers beyond the reference (which disclaims them,
+ * cbct_backprojector/filtering.py:1-8, docs/USAGE.md:332):
+ * det_u0/det_v0 (both or NULL): centre the cosine preweight on each view's
+ * detector centre instead of the detector's middle; angles (or NULL): Parker
+ * short-scan weights (angles increasing, angles[A-1] - angles[0] > pi; delta =
+ * (range - pi)/2).  cbct_fdk_reconstruct_ex: short_scan applies Parker weights
+ * and weights each view by (angles[A-1] - angles[0]) / (A - 1) instead of pi/A
+ * (it reads the two end angles and synchronises the stream once);
+ * offset_preweight uses det_u0/det_v0 in the preweight. */
+int cbct_fdk_filter_ex(const float* projections, int64_t A, int64_t U, int64_t V,
+                       float pixel_size, const float* sod, const float* sid, float alpha,
+                       const float* det_u0, const float* det_v0, const float* angles,
+                       float* filtered, void* workspace, size_t workspace_bytes, void* stream);
+int cbct_fdk_reconstruct_ex(const float* projections, int64_t A, int64_t U, int64_t V,
+                            const float* angles, const float* sid, const float* sod,
+                            const float* det_u0, const float* det_v0,
+                            const float* src_x, const float* src_y, const float* src_z,
+                            float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
+                            float off_x, float off_y, float off_z,
+                            float alpha, int64_t angle_chunk_size, int short_scan, int offset_preweight,
+                            float* volume, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---- geometry calibration ------------------------------------------------
  * No reference counterpart: the reference leaves the calibration loop to user
  * code (examples/autograd.py:62-63, README.md:8).  Fits the eight per-view

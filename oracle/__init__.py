@@ -234,20 +234,47 @@ def filter_taps(U, pixel_size, alpha=0.8):
     return hc[d % pad]
 
 
-def _preweight(projections, pixel_size, sod, sid):
+def parker_weights(angles, U, pixel_size, sid, u0=None):
+    """Parker short-scan weights [A, U] (Parker 1982; Kak & Slaney ch. 3) for fan angle
+    gamma = atan(u_p / sid) of each column, beta = angle - angle_0 and delta = (range - pi)/2;
+    the conjugate of the ray (beta, gamma) in this geometry is (beta + pi + 2 gamma, -gamma)."""
+    ang = np.asarray(angles, np.float64).reshape(-1)
+    A = ang.size
+    beta = (ang - ang[0])[:, None]
+    delta = max((ang[-1] - ang[0] - math.pi) / 2.0, 1e-9)
+    cu = np.full(A, (U - 1) / 2.0) if u0 is None else np.asarray(u0, np.float64).reshape(-1)
+    g = np.arctan((np.arange(U)[None, :] - cu[:, None]) * pixel_size / _per_view(sid, A)[:, None])
+    w = np.zeros((A, U))
+    r1 = (beta >= 0) & (beta < 2 * delta - 2 * g)
+    r2 = (beta >= 2 * delta - 2 * g) & (beta <= math.pi - 2 * g)
+    r3 = (beta > math.pi - 2 * g) & (beta <= math.pi + 2 * delta)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        w = np.where(r1, np.sin(math.pi / 4 * beta / (delta - g)) ** 2, w)
+        w = np.where(r2, 1.0, w)
+        w = np.where(r3, np.sin(math.pi / 4 * (math.pi + 2 * delta - beta) / (delta + g)) ** 2, w)
+    return w
+
+
+def _preweight(projections, pixel_size, sod, sid, u0=None, v0=None, parker_angles=None):
     A, U, V = projections.shape
     sid = _per_view(sid, A)[:, None, None]
-    u = (np.arange(U) - (U - 1) / 2.0) * pixel_size
-    v = (np.arange(V) - (V - 1) / 2.0) * pixel_size
-    dist = np.sqrt(sid ** 2 + u[None, :, None] ** 2 + v[None, None, :] ** 2)
-    return np.asarray(projections, np.float64) * (sid / dist)
+    cu = np.full(A, (U - 1) / 2.0) if u0 is None else np.asarray(u0, np.float64).reshape(-1)
+    cv = np.full(A, (V - 1) / 2.0) if v0 is None else np.asarray(v0, np.float64).reshape(-1)
+    u = (np.arange(U)[None, :] - cu[:, None]) * pixel_size
+    v = (np.arange(V)[None, :] - cv[:, None]) * pixel_size
+    dist = np.sqrt(sid ** 2 + u[:, :, None] ** 2 + v[:, None, :] ** 2)
+    w = np.asarray(projections, np.float64) * (sid / dist)
+    if parker_angles is not None:
+        w = w * parker_weights(parker_angles, U, pixel_size, sid[:, 0, 0], u0)[:, :, None]
+    return w
 
 
-def fdk_filter(projections, pixel_size, sod, sid, alpha=0.8):
-    """FP64 FFT restatement of fdk_filter (filtering.py:54-104)."""
+def fdk_filter(projections, pixel_size, sod, sid, alpha=0.8, u0=None, v0=None, parker_angles=None):
+    """FP64 FFT restatement of fdk_filter (filtering.py:54-104); u0/v0 centre the preweight
+    per view and parker_angles apply short-scan weights (extensions beyond the reference)."""
     p = np.asarray(projections, np.float64)
     A, U, V = p.shape
-    w = _preweight(p, pixel_size, sod, sid)
+    w = _preweight(p, pixel_size, sod, sid, u0, v0, parker_angles)
     pad = fdk_pad(U)
     F = np.fft.fft(w, n=pad, axis=1) * ramp_hann_response(U, pixel_size, alpha)[None, :, None]
     out = np.fft.ifft(F, axis=1).real[:, :U, :]
@@ -271,14 +298,19 @@ def fdk_filter_direct(projections, pixel_size, sod, sid, alpha=0.8):
 
 
 def reconstruct_fdk(projections, geometry, pixel_size, volume_shape, voxel_size,
-                    offset=(0.0, 0.0, 0.0), alpha=0.8, angle_chunk_size=32, mode="f64"):
-    """filter -> chunked weighted backprojection -> x pi/A -> clamp at 0 (filtering.py:182-198)."""
+                    offset=(0.0, 0.0, 0.0), alpha=0.8, angle_chunk_size=32, mode="f64",
+                    short_scan=False, offset_preweight=False):
+    """filter -> chunked weighted backprojection -> x pi/A -> clamp at 0 (filtering.py:182-198);
+    short_scan: Parker weights and x (range / (A - 1)) instead of pi/A."""
     g = _geom(geometry, np.float64)
     A = g[0].size
-    filt = fdk_filter(projections, pixel_size, g[2], g[1], alpha)
+    filt = fdk_filter(projections, pixel_size, g[2], g[1], alpha,
+                      u0=g[3] if offset_preweight else None, v0=g[4] if offset_preweight else None,
+                      parker_angles=g[0] if short_scan else None)
     vol = bp_forward(filt, g, pixel_size, volume_shape, voxel_size, offset,
                      chunk=angle_chunk_size, mode=mode)
-    vol = vol * (math.pi / A)
+    scale = (g[0][-1] - g[0][0]) / (A - 1) if short_scan else math.pi / A
+    vol = vol * scale
     return np.maximum(vol, 0.0)
 
 

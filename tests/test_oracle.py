@@ -353,3 +353,66 @@ def test_preprocess_oracle_recovers_line_integrals_and_transposes():
     got2 = oracle.preprocess(raw2, dark, flat2, "AVU", min_transmission=1e-4)
     assert got2[0, 0, 0] == -np.log(1e-4) and got2[0, 2, 1] == -np.log(1e-4)
     assert np.array_equal(oracle.preprocess(raw, layout="AUV", log=False), raw)
+
+
+def test_parker_weights_sum_to_one_over_conjugate_rays():
+    """Every line measured twice in a short scan gets total weight one: the conjugate of the ray
+    (beta, gamma) is (beta + pi + 2 gamma, -gamma) in this geometry (checked from the ray
+    definition itself), and the two Parker weights add up to 1."""
+    sod, sid, pix, U = 500.0, 1000.0, 1.0, 201
+    # the conjugate relation, from the source/detector construction of the projector
+    beta, up = 0.3, 60.0
+    s, c = math.sin(beta), math.cos(beta)
+    O = np.array([-sod * s, sod * c])
+    T = np.array([up * c - (sod - sid) * s, up * s + (sod - sid) * c])
+    d = (T - O) / np.linalg.norm(T - O)
+    P2 = O - 2 * O.dot(d) * d
+    beta2 = math.atan2(-P2[0], P2[1])
+    gamma = math.atan(up / sid)
+    assert abs(((beta2 - beta) - (math.pi + 2 * gamma)) % (2 * math.pi)) < 1e-9
+    # redundancy of the weights: evaluate each ray and its conjugate exactly (a three-view scan
+    # [0, beta, range] keeps delta and places the view of interest at beta)
+    delta = math.atan(((U - 1) / 2) * pix / sid)
+    rng_ = math.pi + 2 * delta
+    g = np.arctan((np.arange(U) - (U - 1) / 2) * pix / sid)
+    weight = lambda b, col: oracle.parker_weights(np.array([0.0, b, rng_]), U, pix, sid)[1, col]
+    w_all = oracle.parker_weights(np.linspace(0.0, rng_, 361), U, pix, sid)
+    assert w_all.min() >= 0.0 and w_all.max() <= 1.0 and np.all(w_all[0] == 0.0)
+    rs = np.random.default_rng(0)
+    checked = 0
+    for _ in range(400):
+        b, u = rs.uniform(0.0, rng_), int(rs.integers(U))
+        b2 = b + math.pi + 2 * g[u]
+        if b2 > rng_:
+            b2 -= 2 * math.pi
+        if not (0.0 <= b2 <= rng_):
+            assert weight(b, u) == 1.0                # measured once: full weight
+            continue
+        u2 = (U - 1) - u                             # the column of fan angle -gamma
+        assert abs(weight(b, u) + weight(b2, u2) - 1.0) < 1e-9
+        checked += 1
+    assert checked > 30
+
+
+def test_short_scan_fdk_reconstructs_centred_sphere():
+    A, U, V, pix, sod, sid = 120, 96, 96, 1.0, 500.0, 1000.0
+    radius = 10.0
+    delta = math.atan(((U - 1) / 2) * pix / sid)
+    ang = np.linspace(0.0, math.pi + 2 * delta, A)
+    proj = sphere_projections(A, U, V, pix, sod, sid, radius)
+    geom = [ang, np.full(A, sid), np.full(A, sod), np.full(A, (U - 1) / 2), np.full(A, (V - 1) / 2),
+            np.zeros(A), np.zeros(A), np.zeros(A)]
+    vol = oracle.reconstruct_fdk(proj, geom, pix, (48, 48, 48), 0.5, short_scan=True)
+    z, y, x = np.meshgrid(*(np.arange(48) - 24,) * 3, indexing="ij")
+    r = np.sqrt(x * x + y * y + z * z) * 0.5
+    assert abs(vol[r < 7.0].mean() - 1.0) < 0.03
+    assert vol[r > 13.0].mean() < 0.03
+
+
+def test_offset_preweight_at_the_centre_equals_the_reference_preweight():
+    rng = np.random.default_rng(9)
+    A, U, V = 3, 20, 14
+    p = rng.random((A, U, V))
+    ref = oracle.fdk_filter(p, 0.7, 300.0, 500.0)
+    got = oracle.fdk_filter(p, 0.7, 300.0, 500.0, u0=np.full(A, (U - 1) / 2), v0=np.full(A, (V - 1) / 2))
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)

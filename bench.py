@@ -43,6 +43,9 @@ WORKLOADS[CALIB_WORKLOAD] = dict(problems=64, adam_steps=20, side=(96, 320), vie
 WARM_WORKLOAD = "calibration-time-256c-180v"
 WORKLOADS[WARM_WORKLOAD] = dict(train_steps=1500, pool=24, batch=2, scans=(7001, 7002, 7003, 7004), threshold=0.1,
                                 adam_budget=150, lm_budget=20, plain_adam_lr=4.0, warm_adam_lr=1.0, lm_lambda=1e-3)
+BP_WORKLOAD = "backprojection-256c-360v"
+WORKLOADS[BP_WORKLOAD] = dict(D=256, A=360, U=512, V=384, pixel=0.8, sod=500.0, sid=1000.0, voxel=0.5, step=0.25,
+                              perturb=True)
 PREP_WORKLOAD = "preprocess-720v-768x640-u16"
 WORKLOADS[PREP_WORKLOAD] = dict(A=720, U=768, V=640)
 DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
@@ -342,6 +345,57 @@ def warm_start_bench(args, torch, device):
     print(json.dumps(out), flush=True)
 
 
+def backprojection_bench(args, torch, device):
+    """Weighted backprojection (R6) and its backward (R7) at 256^3 x 360 views, 512x384 detector,
+    perturbed geometry.  One step = one backproject + one backward (projection gradient and the
+    eight geometry gradients) for a fixed upstream volume gradient.  Algorithmic bytes per
+    voxel-view: forward 16 (four taps); backward 16 (taps) + 16 (four scattered adds)."""
+    from cbct_backprojector import _native
+    w = WORKLOADS[BP_WORKLOAD]
+    _, _, vol, geom, _ = make_inputs(w, device)
+    D, A, U, V = w["D"], w["A"], w["U"], w["V"]
+    scal = (w["pixel"], U, V, w["step"], w["voxel"], 0.0, 0.0, 0.0)
+    proj = _native.fp_forward(vol, geom, *scal)
+    gen = torch.Generator(device=device).manual_seed(12)
+    gvol = torch.randn((D, D, D), generator=gen, device=device)
+
+    def step(ev=None):
+        if ev:
+            ev[0].record()
+        out = _native.bp_forward(proj, geom, w["pixel"], D, D, D, w["voxel"], 0.0, 0.0, 0.0)
+        if ev:
+            ev[1].record()
+        back = _native.bp_backward(gvol, proj, geom, w["pixel"], w["voxel"], 0.0, 0.0, 0.0)
+        if ev:
+            ev[2].record()
+        return out, back
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_bp = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps / 1e3
+    t_bb = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps / 1e3
+    N = D ** 3 * A
+    b_bp, b_bb = 16.0 * N + 4.0 * D ** 3, 32.0 * N + 4.0 * D ** 3
+    dom = ("bp_backward", b_bb, t_bb) if t_bb >= t_bp else ("bp_forward", b_bp, t_bp)
+    achieved = dom[1] / dom[2] / 1e9
+    print(json.dumps({
+        "metric": METRIC, "value": 2.0 * N / (t_bp + t_bb) / 1e9, "unit": "GUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": BP_WORKLOAD, "volume": [D] * 3, "views": A, "detector": [U, V],
+                   "t_bp_ms": t_bp * 1e3, "t_bp_backward_ms": t_bb * 1e3},
+        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
+    }), flush=True)
+
+
 def preprocess_bench(args, torch, device):
     """Raw uint16 frames [720, 640, 768] ([A,V,U]) + dark/flat -> line integrals [720, 768, 640]
     fp32 (dark/flat correction, -log, transpose) in one kernel; HBM-bound.  Algorithmic bytes:
@@ -398,6 +452,8 @@ def main():
         return warm_start_bench(args, torch, device)
     if args.workload == PREP_WORKLOAD:
         return preprocess_bench(args, torch, device)
+    if args.workload == BP_WORKLOAD:
+        return backprojection_bench(args, torch, device)
     if args.fp_tuning:
         _native.set_fp_tuning(*[int(x) for x in args.fp_tuning.split(",")])
     if args.deterministic:

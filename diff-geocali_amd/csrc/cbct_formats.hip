@@ -14,26 +14,12 @@ namespace {
 constexpr int PT = 64;              // tile edge
 constexpr int PT_PAD = PT + 1;      // LDS row stride (odd: transposed reads hit distinct banks)
 
-constexpr int PV = 1;               // views per workgroup (more views reuse the dark/flat terms but cost parallelism)
-
-// Per-pixel correction terms kept in registers across the workgroup's views:
-// p = -log(max((x - d) * r, tmin)) with r = 1 / (flat - dark), or the clamp value for a dead pixel.
-struct PixelTerms {
-    float d, r;
-    bool dead;
-};
-
-__device__ __forceinline__ PixelTerms pixel_terms(const float* dark, const float* flat, int64_t pix) {
-    PixelTerms t;
-    t.d = dark ? dark[pix] : 0.f;
-    const float den = (flat ? flat[pix] : 1.f) - t.d;
-    t.dead = !(den > 0.f);
-    t.r = t.dead ? 0.f : 1.0f / den;
-    return t;
-}
-
-__device__ __forceinline__ float line_integral(float x, const PixelTerms& t, float tmin, float log_tmin) {
-    return t.dead ? -log_tmin : -__logf(fmaxf((x - t.d) * t.r, tmin));
+// -log of the transmission of one pixel; a pixel with flat <= dark gives -log(t_min)
+__device__ __forceinline__ float line_integral(float x, const float* dark, const float* flat, int64_t pix,
+                                               float tmin, float log_tmin) {
+    const float d = dark ? dark[pix] : 0.f;
+    const float den = (flat ? flat[pix] : 1.f) - d;
+    return den > 0.f ? -__logf(fmaxf((x - d) / den, tmin)) : -log_tmin;
 }
 
 template <typename T, int TRANSPOSE>
@@ -43,47 +29,38 @@ __global__ __launch_bounds__(256) void preprocess_kernel(const T* __restrict__ r
     __shared__ float tile[PT * PT_PAD];
     const int tiles_u = (U + PT - 1) / PT;
     const int tu = blockIdx.x % tiles_u, tv = blockIdx.x / tiles_u;
+    const int64_t a = blockIdx.y;
     const int u0 = tu * PT, v0 = tv * PT;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;      // 64 lanes along the fast axis, 4 rows
     const float log_tmin = __logf(tmin);
-    // this thread's 16 pixels: rows ty, ty+4, ..., fast index tx
-    PixelTerms terms[PT / 4];
+    const T* src = raw + a * V * U;
     const int f = u0 + tx;
-#pragma unroll
-    for (int k = 0; k < PT / 4; ++k) {
-        const int s = v0 + ty + 4 * k;
-        terms[k] = (take_log && s < V && f < U) ? pixel_terms(dark, flat, (int64_t)s * U + f) : PixelTerms{0.f, 1.f, false};
-    }
-    const int64_t a_end = min(A, (int64_t)(blockIdx.y + 1) * PV);
-    for (int64_t a = (int64_t)blockIdx.y * PV; a < a_end; ++a) {
-        const T* src = raw + a * V * U;
-        if (TRANSPOSE) {
-#pragma unroll
-            for (int k = 0; k < PT / 4; ++k) {
-                const int r = ty + 4 * k, s = v0 + r;
-                float p = 0.f;
-                if (s < V && f < U) {
-                    const float x = (float)src[(int64_t)s * U + f];
-                    p = take_log ? line_integral(x, terms[k], tmin, log_tmin) : x;
-                }
-                tile[r * PT_PAD + tx] = p;
+    if (TRANSPOSE) {
+#pragma unroll 4
+        for (int r = ty; r < PT; r += 4) {
+            const int s = v0 + r;
+            float p = 0.f;
+            if (s < V && f < U) {
+                const int64_t pix = (int64_t)s * U + f;
+                const float x = (float)src[pix];
+                p = take_log ? line_integral(x, dark, flat, pix, tmin, log_tmin) : x;
             }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < PT / 4; ++k) {
-                const int r = ty + 4 * k, u = u0 + r, v = v0 + tx;
-                if (u < U && v < V) out[(a * U + u) * V + v] = tile[tx * PT_PAD + r];
-            }
-            __syncthreads();
-        } else {
-#pragma unroll
-            for (int k = 0; k < PT / 4; ++k) {
-                const int s = v0 + ty + 4 * k;
-                if (s < V && f < U) {
-                    const int64_t pix = (int64_t)s * U + f;
-                    const float x = (float)src[pix];
-                    out[a * V * U + pix] = take_log ? line_integral(x, terms[k], tmin, log_tmin) : x;
-                }
+            tile[r * PT_PAD + tx] = p;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int r = ty; r < PT; r += 4) {
+            const int u = u0 + r, v = v0 + tx;
+            if (u < U && v < V) out[(a * U + u) * V + v] = tile[tx * PT_PAD + r];
+        }
+    } else {
+#pragma unroll 4
+        for (int r = ty; r < PT; r += 4) {
+            const int s = v0 + r;
+            if (s < V && f < U) {
+                const int64_t pix = (int64_t)s * U + f;
+                const float x = (float)src[pix];
+                out[a * V * U + pix] = take_log ? line_integral(x, dark, flat, pix, tmin, log_tmin) : x;
             }
         }
     }
@@ -106,7 +83,7 @@ extern "C" int cbct_preprocess(const void* raw, int raw_dtype, const float* dark
     CBCT_CHECK_ARG(min_transmission > 0.f && min_transmission <= 1.f, "min_transmission must be in (0, 1]");
     const int64_t tiles = ceil_div(rows, (int64_t)PT) * ceil_div(cols, (int64_t)PT);
     CBCT_CHECK_ARG(tiles <= INT32_MAX, "too many tiles");
-    const dim3 grid((unsigned)tiles, (unsigned)ceil_div(A, (int64_t)PV));
+    const dim3 grid((unsigned)tiles, (unsigned)A);
     hipStream_t s = (hipStream_t)stream;
     const int R = (int)rows, Cc = (int)cols, lg = take_log ? 1 : 0;
     if (raw_dtype == 0) {

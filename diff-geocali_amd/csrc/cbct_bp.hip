@@ -96,95 +96,214 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_forward_kernel(
 // Backward: per voxel-view, the bilinear transpose scatters into gproj and the
 // analytic derivatives of weight*interp feed the eight geometry sums, which
 // are reduced over the wave and added once per wave and view.
+// Backward: the transpose of the bilinear interpolation (the projection gradient) and the
+// eight per-view geometry derivatives (csrc/src_cbct_backprojector_cuda.cu:101-295).
+//
+// One workgroup owns an 8x8 block of voxel columns and 64*K voxels of each along z; each wave
+// takes 16 of the block's columns (lane = z).  Per view the block's taps fall in a small
+// detector window (columns of the block share a few detector rows u, and a column covers a run
+// of v): the workgroup bounds that window exactly (same float expressions as the taps), clears
+// an LDS tile for it, adds its projection-gradient contributions there as 32-bit fixed-point
+// integers (ds_add_u32; float LDS atomics are ~40x slower on gfx950), and flushes the tile with
+// one coalesced float atomic per touched detector cell -- about an order of magnitude fewer
+// global atomics than four per voxel-view.  A window larger than the tile falls back to direct
+// global atomics for that view.  The fixed-point scale 2^e per view keeps the largest possible
+// cell total (max|grad| x max weight x columns x voxels of a column per cell) below 2^30.
+constexpr int BT = 8;                          // column block edge
+constexpr int BT_COLS_PER_WAVE = BT * BT / BP_WAVES;
+constexpr int BT_CAP = 8192;                   // LDS tile cells
+
+struct BpColumn {
+    float px, py, m, q, l, iq, iq2, iq3, t, du, gw, ui;
+    bool ok;
+};
+
+__device__ __forceinline__ BpColumn bp_column(const ViewFrame& f, float vx, float vy, float ip) {
+    BpColumn c;
+    c.px = fmaf(vx, f.c, vy * f.s);
+    c.py = fmaf(-vx, f.s, vy * f.c);
+    c.m = f.sod + f.dsn;
+    c.q = c.m - c.py;
+    c.l = f.sid + f.dsn;
+    c.ok = !(c.q <= 1e-6f || c.l <= 1e-6f);
+    c.iq = 1.0f / c.q;
+    c.iq2 = c.iq * c.iq;
+    c.iq3 = c.iq2 * c.iq;
+    c.t = c.l * c.iq;
+    c.du = c.px - f.dsu;
+    const float uph = fmaf(c.t, c.du, f.dsu);
+    c.gw = (c.m * c.m) * c.iq2;
+    c.ui = fmaf(uph, ip, f.u0);
+    return c;
+}
+
+__device__ __forceinline__ float bp_vi(const ViewFrame& f, float t, float vz, float ip) {
+    const float dv = vz - f.dsv;
+    const float vph = fmaf(t, dv, f.dsv);
+    return fmaf(vph, ip, f.v0);
+}
+
 template <int K>
 __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
     const float* __restrict__ gvol, const float* __restrict__ proj,
     const ViewFrame* __restrict__ frames, BpGeom g,
     float* __restrict__ gproj, float* __restrict__ ggeom) {
+    __shared__ unsigned tile[BT_CAP];
+    __shared__ float red[BP_WAVES];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int col = __builtin_amdgcn_readfirstlane(blockIdx.x * BP_WAVES + wave);
-    if (col >= g.H * g.W) return;
-    const int x = col % g.W, y = col / g.W;
-    const int zb = blockIdx.y * 64 * K + lane;
-    const float vx = fmaf((float)x - g.hx, g.vs, g.offx);
-    const float vy = fmaf((float)y - g.hy, g.vs, g.offy);
+    const int nbx = (g.W + BT - 1) / BT;
+    const int bx = blockIdx.x % nbx, by = blockIdx.x / nbx;
+    const int z0 = blockIdx.y * 64 * K;
     const int64_t plane = (int64_t)g.H * g.W;
-    float vz[K], gv[K];
+    float vz[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int z = zb + 64 * k;
-        vz[k] = fmaf((float)z - g.hz, g.vs, g.offz);
-        gv[k] = z < g.D ? gvol[z * plane + (int64_t)y * g.W + x] : 0.0f;
-    }
-    const float Um1 = (float)g.U - 1.0f, Vm1 = (float)g.V - 1.0f;
-    const float ip = g.inv_pix;
-    for (int a = 0; a < g.A; ++a) {
-        const ViewFrame f = frames[a];
-        const float px = fmaf(vx, f.c, vy * f.s);
-        const float py = fmaf(-vx, f.s, vy * f.c);
-        const float m = f.sod + f.dsn;
-        const float q = m - py;
-        const float l = f.sid + f.dsn;
-        if (q <= 1e-6f || l <= 1e-6f) continue;
-        const float iq = 1.0f / q, iq2 = iq * iq, iq3 = iq2 * iq;
-        const float t = l * iq;
-        const float du = px - f.dsu;
-        const float uph = fmaf(t, du, f.dsu);
-        const float m2 = m * m;
-        const float gw = m2 * iq2;
-        const float ui = fmaf(uph, ip, f.u0);
-        if (!(ui >= 0.0f && ui < Um1)) continue;
-        const int iu = (int)floorf(ui);
-        const float fu = ui - (float)iu, mu = 1.0f - fu;
-        const int64_t base = ((int64_t)a * g.U + iu) * g.V;
-        // z-independent derivative factors
-        const float dt_sod = -l * iq2;
-        const float dW_m = 2.0f * m * iq2 - 2.0f * m2 * iq3;     // d(m^2/q^2)/dm at fixed q-m
-        const float dsu_b = f.dsn, dsn_b = -f.dsu;
-        const float dq_b = dsn_b + px;
-        const float dt_b = (dsn_b * q - l * dq_b) * iq2;
-        const float dU_b = fmaf(du, dt_b, fmaf(1.0f - t, dsu_b, t * py));
-        const float dW_b = 2.0f * m * iq2 * dsn_b - 2.0f * m2 * iq3 * dq_b;
-        const float qml = q - l;
-        const float dt_sx = -f.s * qml * iq2;
-        const float dU_sx = fmaf(du, dt_sx, (1.0f - t) * f.c);
-        const float dW_sx = dW_m * -f.s;
-        const float dt_sy = f.c * qml * iq2;
-        const float dU_sy = fmaf(du, dt_sy, (1.0f - t) * f.s);
-        const float dW_sy = dW_m * f.c;
-        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f;
+    for (int k = 0; k < K; ++k) vz[k] = fmaf((float)(z0 + lane + 64 * k) - g.hz, g.vs, g.offz);
+    // the wave's columns: block column c = wave + BP_WAVES * j
+    float gv[BT_COLS_PER_WAVE][K];
+    float amax = 0.f;
+#pragma unroll
+    for (int j = 0; j < BT_COLS_PER_WAVE; ++j) {
+        const int c = wave + BP_WAVES * j;
+        const int x = bx * BT + c % BT, y = by * BT + c / BT;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const float dv = vz[k] - f.dsv;
-            const float vph = fmaf(t, dv, f.dsv);
-            const float vi = fmaf(vph, ip, f.v0);
-            if (zb + 64 * k < g.D && vi >= 0.0f && vi < Vm1) {
-                const int iv = (int)floorf(vi);
-                const float fv = vi - (float)iv, mv = 1.0f - fv;
-                const int64_t k00 = base + iv;
-                const float p00 = proj[k00], p01 = proj[k00 + 1];
-                const float p10 = proj[k00 + g.V], p11 = proj[k00 + g.V + 1];
-                const float interp = fmaf(fu * fv, p11, fmaf(mu * fv, p01, fmaf(fu * mv, p10, mu * mv * p00)));
-                const float Iu = fmaf(fv, p11 - p01, mv * (p10 - p00));
-                const float Iv = fmaf(fu, p11 - p10, mu * (p01 - p00));
-                const float gb = gv[k] * gw;
-                atomicAdd(gproj + k00, gb * mu * mv);
-                atomicAdd(gproj + k00 + g.V, gb * fu * mv);
-                atomicAdd(gproj + k00 + 1, gb * mu * fv);
-                atomicAdd(gproj + k00 + g.V + 1, gb * fu * fv);
-                // sid (:214-219)
-                s1 += gb * fmaf(Iv, dv * iq * ip, Iu * du * iq * ip);
-                // sod (:222-230)
-                s2 += gv[k] * fmaf(gw, fmaf(Iv, dv * dt_sod * ip, Iu * du * dt_sod * ip), dW_m * interp);
-                // beta (:233-250)
-                s0 += gv[k] * fmaf(gw, fmaf(Iv, dv * dt_b * ip, Iu * dU_b * ip), dW_b * interp);
-                // detector centres (:252-253)
-                s3 += gb * Iu;
-                s4 += gb * Iv;
-                // source offsets (:255-292)
-                s5 += gv[k] * fmaf(gw, fmaf(Iv, dv * dt_sx * ip, Iu * dU_sx * ip), dW_sx * interp);
-                s6 += gv[k] * fmaf(gw, fmaf(Iv, dv * dt_sy * ip, Iu * dU_sy * ip), dW_sy * interp);
-                s7 += gb * Iv * (1.0f - t) * ip;
+            const int z = z0 + lane + 64 * k;
+            gv[j][k] = (x < g.W && y < g.H && z < g.D) ? gvol[z * plane + (int64_t)y * g.W + x] : 0.0f;
+            amax = fmaxf(amax, fabsf(gv[j][k]));
+        }
+    }
+    for (int s = 32; s >= 1; s >>= 1) amax = fmaxf(amax, __shfl_xor(amax, s, 64));
+    if (lane == 0) red[wave] = amax;
+    __syncthreads();
+    amax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    // the block column of this lane for the window bounds, and the chunk's z extremes
+    const int bcx = bx * BT + lane % BT, bcy = by * BT + lane / BT;
+    const bool bcol = bcx < g.W && bcy < g.H;
+    const float bvx = fmaf((float)bcx - g.hx, g.vs, g.offx), bvy = fmaf((float)bcy - g.hy, g.vs, g.offy);
+    const int zhi = min(z0 + 64 * K, g.D) - 1;
+    const float vz_lo = fmaf((float)z0 - g.hz, g.vs, g.offz), vz_hi = fmaf((float)zhi - g.hz, g.vs, g.offz);
+    const float Um1 = (float)g.U - 1.0f, Vm1 = (float)g.V - 1.0f;
+    const float ip = g.inv_pix;
+    const float INF = __builtin_huge_valf();
+
+    for (int a = 0; a < g.A; ++a) {
+        const ViewFrame f = frames[a];
+        // ---- exact detector window of the block's taps (uniform over the workgroup)
+        float umin = INF, umax = -INF, vmin = INF, vmax = -INF, gwmax = 0.f, tmin = INF;
+        if (bcol) {
+            const BpColumn c = bp_column(f, bvx, bvy, ip);
+            if (c.ok) {
+                const float v1 = bp_vi(f, c.t, vz_lo, ip), v2 = bp_vi(f, c.t, vz_hi, ip);
+                umin = umax = c.ui;
+                vmin = fminf(v1, v2);
+                vmax = fmaxf(v1, v2);
+                gwmax = c.gw;
+                tmin = c.t;
+            }
+        }
+        for (int s = 32; s >= 1; s >>= 1) {
+            umin = fminf(umin, __shfl_xor(umin, s, 64));
+            umax = fmaxf(umax, __shfl_xor(umax, s, 64));
+            vmin = fminf(vmin, __shfl_xor(vmin, s, 64));
+            vmax = fmaxf(vmax, __shfl_xor(vmax, s, 64));
+            gwmax = fmaxf(gwmax, __shfl_xor(gwmax, s, 64));
+            tmin = fminf(tmin, __shfl_xor(tmin, s, 64));
+        }
+        if (!(umax >= 0.f && umin < Um1 && vmax >= 0.f && vmin < Vm1)) continue;   // no tap on the detector
+        const int u_lo = (int)fmaxf(floorf(umin), 0.f), u_hi = (int)fminf(floorf(umax), (float)(g.U - 2));
+        const int v_lo = (int)fmaxf(floorf(vmin), 0.f), v_hi = (int)fminf(floorf(vmax), (float)(g.V - 2));
+        const int nu = u_hi - u_lo + 2, nv = v_hi - v_lo + 2;
+        const bool staged = nu * nv <= BT_CAP;
+        float scale = 1.f, inv_scale = 1.f;
+        if (staged) {
+            const float spacing = g.vs * tmin * ip;                  // detector rows per voxel along z
+            const float nz = spacing > 0.f ? fminf(floorf(2.0f / spacing) + 2.0f, 64.0f * K) : 64.0f * K;
+            const float bound = amax * gwmax * (float)(BT * BT) * nz;
+            int e = 0;
+            if (bound > 0.f) {
+                int k2;
+                frexpf(1073741824.f / bound, &k2);
+                e = max(-100, min(100, k2 - 1));
+            }
+            scale = ldexpf(1.f, e);
+            inv_scale = ldexpf(1.f, -e);
+            for (int i = threadIdx.x; i < nu * nv; i += 64 * BP_WAVES) tile[i] = 0u;
+        }
+        __syncthreads();
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f;
+#pragma unroll 1
+        for (int j = 0; j < BT_COLS_PER_WAVE; ++j) {
+            const int cidx = wave + BP_WAVES * j;
+            const int x = bx * BT + cidx % BT, y = by * BT + cidx / BT;
+            if (x >= g.W || y >= g.H) continue;
+            const float vx = fmaf((float)x - g.hx, g.vs, g.offx);
+            const float vy = fmaf((float)y - g.hy, g.vs, g.offy);
+            const BpColumn c = bp_column(f, vx, vy, ip);
+            if (!c.ok) continue;
+            if (!(c.ui >= 0.0f && c.ui < Um1)) continue;
+            const float iq = c.iq, iq2 = c.iq2, iq3 = c.iq3, t = c.t, du = c.du, gw = c.gw;
+            const float m = c.m, l = c.l, q = c.q, px = c.px, py = c.py;
+            const float m2 = m * m;
+            const int iu = (int)floorf(c.ui);
+            const float fu = c.ui - (float)iu, mu = 1.0f - fu;
+            const int64_t base = ((int64_t)a * g.U + iu) * g.V;
+            // z-independent derivative factors
+            const float dt_sod = -l * iq2;
+            const float dW_m = 2.0f * m * iq2 - 2.0f * m2 * iq3;     // d(m^2/q^2)/dm at fixed q-m
+            const float dsu_b = f.dsn, dsn_b = -f.dsu;
+            const float dq_b = dsn_b + px;
+            const float dt_b = (dsn_b * q - l * dq_b) * iq2;
+            const float dU_b = fmaf(du, dt_b, fmaf(1.0f - t, dsu_b, t * py));
+            const float dW_b = 2.0f * m * iq2 * dsn_b - 2.0f * m2 * iq3 * dq_b;
+            const float qml = q - l;
+            const float dt_sx = -f.s * qml * iq2;
+            const float dU_sx = fmaf(du, dt_sx, (1.0f - t) * f.c);
+            const float dW_sx = dW_m * -f.s;
+            const float dt_sy = f.c * qml * iq2;
+            const float dU_sy = fmaf(du, dt_sy, (1.0f - t) * f.s);
+            const float dW_sy = dW_m * f.c;
+            const int row = (iu - u_lo) * nv;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const float dv = vz[k] - f.dsv;
+                const float vph = fmaf(t, dv, f.dsv);
+                const float vi = fmaf(vph, ip, f.v0);
+                if (z0 + lane + 64 * k < g.D && vi >= 0.0f && vi < Vm1) {
+                    const int iv = (int)floorf(vi);
+                    const float fv = vi - (float)iv, mv = 1.0f - fv;
+                    const int64_t k00 = base + iv;
+                    const float p00 = proj[k00], p01 = proj[k00 + 1];
+                    const float p10 = proj[k00 + g.V], p11 = proj[k00 + g.V + 1];
+                    const float interp = fmaf(fu * fv, p11, fmaf(mu * fv, p01, fmaf(fu * mv, p10, mu * mv * p00)));
+                    const float Iu = fmaf(fv, p11 - p01, mv * (p10 - p00));
+                    const float Iv = fmaf(fu, p11 - p10, mu * (p01 - p00));
+                    const float gb = gv[j][k] * gw;
+                    if (staged) {
+                        unsigned* q0 = tile + row + (iv - v_lo);
+                        atomicAdd(q0, (unsigned)__float2int_rn(gb * mu * mv * scale));
+                        atomicAdd(q0 + nv, (unsigned)__float2int_rn(gb * fu * mv * scale));
+                        atomicAdd(q0 + 1, (unsigned)__float2int_rn(gb * mu * fv * scale));
+                        atomicAdd(q0 + nv + 1, (unsigned)__float2int_rn(gb * fu * fv * scale));
+                    } else {
+                        atomicAdd(gproj + k00, gb * mu * mv);
+                        atomicAdd(gproj + k00 + g.V, gb * fu * mv);
+                        atomicAdd(gproj + k00 + 1, gb * mu * fv);
+                        atomicAdd(gproj + k00 + g.V + 1, gb * fu * fv);
+                    }
+                    // sid (:214-219)
+                    s1 += gb * fmaf(Iv, dv * iq * ip, Iu * du * iq * ip);
+                    // sod (:222-230)
+                    s2 += gv[j][k] * fmaf(gw, fmaf(Iv, dv * dt_sod * ip, Iu * du * dt_sod * ip), dW_m * interp);
+                    // beta (:233-250)
+                    s0 += gv[j][k] * fmaf(gw, fmaf(Iv, dv * dt_b * ip, Iu * dU_b * ip), dW_b * interp);
+                    // detector centres (:252-253)
+                    s3 += gb * Iu;
+                    s4 += gb * Iv;
+                    // source offsets (:255-292)
+                    s5 += gv[j][k] * fmaf(gw, fmaf(Iv, dv * dt_sx * ip, Iu * dU_sx * ip), dW_sx * interp);
+                    s6 += gv[j][k] * fmaf(gw, fmaf(Iv, dv * dt_sy * ip, Iu * dU_sy * ip), dW_sy * interp);
+                    s7 += gb * Iv * (1.0f - t) * ip;
+                }
             }
         }
         s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
@@ -195,6 +314,17 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
             atomicAdd(ggeom + 4 * g.A + a, s4); atomicAdd(ggeom + 5 * g.A + a, s5);
             atomicAdd(ggeom + 6 * g.A + a, s6); atomicAdd(ggeom + 7 * g.A + a, s7);
         }
+        __syncthreads();
+        if (staged) {
+            for (int r = 0; r < nu; ++r) {
+                float* dst = gproj + ((int64_t)a * g.U + u_lo + r) * g.V + v_lo;
+                for (int cc = threadIdx.x; cc < nv; cc += 64 * BP_WAVES) {
+                    const int iv = (int)tile[r * nv + cc];
+                    if (iv != 0) atomicAdd(dst + cc, (float)iv * inv_scale);
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -303,12 +433,9 @@ extern "C" int cbct_bp_backward(const float* grad_volume, const float* projectio
     if ((st = launch_fill(grad_projections, A * U * V, 0.0f, stream))) return st;
     if ((st = launch_fill(grad_geometry, 8 * A, 0.0f, stream))) return st;
     const BpGeom g = make_bp_geom(A, U, V, D, H, W, pixel_size, voxel_size, off_x, off_y, off_z);
-    const int K = bp_k(D);
-    dim3 grid((unsigned)ceil_div(H * W, BP_WAVES), (unsigned)ceil_div(D, 64 * K));
-    if (K == 4)
-        bp_backward_kernel<4><<<grid, 64 * BP_WAVES, 0, stream>>>(grad_volume, projections, frames, g,
-                                                                 grad_projections, grad_geometry);
-    else if (K == 2)
+    const int K = bp_k(D) >= 2 ? 2 : 1;      // 8x8 columns x 64K voxels per workgroup
+    dim3 grid((unsigned)(ceil_div(W, (int64_t)BT) * ceil_div(H, (int64_t)BT)), (unsigned)ceil_div(D, 64 * K));
+    if (K == 2)
         bp_backward_kernel<2><<<grid, 64 * BP_WAVES, 0, stream>>>(grad_volume, projections, frames, g,
                                                                  grad_projections, grad_geometry);
     else

@@ -1,7 +1,8 @@
 // Weighted voxel-driven backprojector and its backward for gfx950.
 //
-// Work assignment: one wave owns one (x, y) voxel column and 64*K voxels of it
-// along z (lane = z, K voxels per lane).  Along a column only the detector row
+// Work assignment (forward): one wave owns one (x, y) voxel column and 64*K voxels of it
+// along z (lane = z, K voxels per lane); the backward groups 8x8 columns per workgroup
+// (see bp_backward_kernel).  Along a column only the detector row
 // v changes, and v is the contiguous axis of the [A,U,V] projections, so the
 // four bilinear taps of a wave fall in a few cache lines.  All view-dependent
 // terms that do not depend on z (rotation, magnification t, u index, weight)

@@ -134,28 +134,33 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     }
     float gox = 0.f, goy = 0.f, goz = 0.f, gdx = 0.f, gdy = 0.f, gdz = 0.f, gtmin = 0.f;
 
+    // The reciprocal voxel size lives in a VGPR so that fmaf(p - off, inv_vs, half) takes one scalar
+    // operand (half) and needs no per-sample copy of the other.
+    float inv_vs_v = P.inv_vs;
+    asm volatile("" : "+v"(inv_vs_v));
     auto voxel_coords = [&](float& x, float& y, float& z) {
         const float px = fmaf(t, ray.dx, f.ox), py = fmaf(t, ray.dy, f.oy), pz = fmaf(t, ray.dz, f.oz);
         if (MODE == 0 || MODE >= 3) {      // the forward's voxel map (division unless vs is a power of two)
-            x = to_voxel<DIV>(px, b.offx, P.vs, P.inv_vs, b.hx);
-            y = to_voxel<DIV>(py, b.offy, P.vs, P.inv_vs, b.hy);
-            z = to_voxel<DIV>(pz, b.offz, P.vs, P.inv_vs, b.hz);
+            x = to_voxel<DIV>(px, b.offx, P.vs, inv_vs_v, b.hx);
+            y = to_voxel<DIV>(py, b.offy, P.vs, inv_vs_v, b.hy);
+            z = to_voxel<DIV>(pz, b.offz, P.vs, inv_vs_v, b.hz);
         } else {
-            x = fmaf(px - b.offx, P.inv_vs, b.hx);
-            y = fmaf(py - b.offy, P.inv_vs, b.hy);
-            z = fmaf(pz - b.offz, P.inv_vs, b.hz);
+            x = fmaf(px - b.offx, inv_vs_v, b.hx);
+            y = fmaf(py - b.offy, inv_vs_v, b.hy);
+            z = fmaf(pz - b.offz, inv_vs_v, b.hz);
         }
     };
     // Everything a sample does once its eight taps are known.
     auto use_taps = [&](const Taps& tp, float fx, float fy, float fz) {
         if (MODE == 0 || MODE >= 3) integral += tri_value(tp, tri_weights(fx, fy, fz));
         if (MODE != 0) {
+            // Sums of the voxel-unit spatial gradient; the ray's constant factor (upstream
+            // gradient x step / voxel size) is applied once at the end of the ray.
             float sgx, sgy, sgz;
             tri_gradient(tp, fx, fy, fz, sgx, sgy, sgz);
-            const float wx = sgx * P.inv_vs, wy = sgy * P.inv_vs, wz = sgz * P.inv_vs;
-            gox = fmaf(gs, wx, gox); goy = fmaf(gs, wy, goy); goz = fmaf(gs, wz, goz);
-            gdx = fmaf(gs * wx, t, gdx); gdy = fmaf(gs * wy, t, gdy); gdz = fmaf(gs * wz, t, gdz);
-            gtmin = fmaf(gs, fmaf(wz, ray.dz, fmaf(wy, ray.dy, wx * ray.dx)), gtmin);
+            gox += sgx; goy += sgy; goz += sgz;
+            gdx = fmaf(sgx, t, gdx); gdy = fmaf(sgy, t, gdy); gdz = fmaf(sgz, t, gdz);
+            gtmin = fmaf(sgz, ray.dz, fmaf(sgy, ray.dy, fmaf(sgx, ray.dx, gtmin)));
         }
     };
     const int plane = b.H * b.W;
@@ -323,16 +328,19 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                     const int idx = __mul24(__mul24((int)oz, B1) + (int)oy, B0) + (int)ox;
                     const Taps tp = load_taps_strided(vbox, idx, 1, B0, PL);
                     if (MODE == 2) {
-                        const Weights w = tri_weights(fx, fy, fz);
+                        // scaled trilinear weights, the scale folded into the z factors
+                        const float mx = 1.0f - fx, my = 1.0f - fy;
+                        const float g0 = gsq * (1.0f - fz), g1 = gsq * fz;
+                        const float g00 = g0 * my, g01 = g0 * fy, g10 = g1 * my, g11 = g1 * fy;
                         unsigned* qi = gboxu + idx;
-                        atomicAdd(qi, (unsigned)__float2int_rn(gsq * w.w000));
-                        atomicAdd(qi + 1, (unsigned)__float2int_rn(gsq * w.w100));
-                        atomicAdd(qi + B0, (unsigned)__float2int_rn(gsq * w.w010));
-                        atomicAdd(qi + B0 + 1, (unsigned)__float2int_rn(gsq * w.w110));
-                        atomicAdd(qi + PL, (unsigned)__float2int_rn(gsq * w.w001));
-                        atomicAdd(qi + PL + 1, (unsigned)__float2int_rn(gsq * w.w101));
-                        atomicAdd(qi + PL + B0, (unsigned)__float2int_rn(gsq * w.w011));
-                        atomicAdd(qi + PL + B0 + 1, (unsigned)__float2int_rn(gsq * w.w111));
+                        atomicAdd(qi, (unsigned)__float2int_rn(g00 * mx));
+                        atomicAdd(qi + 1, (unsigned)__float2int_rn(g00 * fx));
+                        atomicAdd(qi + B0, (unsigned)__float2int_rn(g01 * mx));
+                        atomicAdd(qi + B0 + 1, (unsigned)__float2int_rn(g01 * fx));
+                        atomicAdd(qi + PL, (unsigned)__float2int_rn(g10 * mx));
+                        atomicAdd(qi + PL + 1, (unsigned)__float2int_rn(g10 * fx));
+                        atomicAdd(qi + PL + B0, (unsigned)__float2int_rn(g11 * mx));
+                        atomicAdd(qi + PL + B0 + 1, (unsigned)__float2int_rn(g11 * fx));
                     }
                     use_taps(tp, fx, fy, fz);
                 } else {
@@ -377,18 +385,23 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         return;
     }
     float loss = 0.f, resid = 0.f;
+    if (MODE == 1 || MODE == 2) {
+        // the ray's upstream weight and the voxel-unit scale
+        const float g = gs * P.inv_vs;
+        gox *= g; goy *= g; goz *= g; gdx *= g; gdy *= g; gdz *= g; gtmin *= g;
+    }
     if (MODE == 4 && inr) {
-        // unit-weight sums times step: the derivatives of this ray's projection value
+        // the derivatives of this ray's projection value: step / voxel size times the sums
         resid = (hit ? integral * P.step : 0.f) - go;
         loss = 0.5f * P.resid_scale * resid * resid;
-        gox *= P.step; goy *= P.step; goz *= P.step; gdx *= P.step; gdy *= P.step; gdz *= P.step;
-        gtmin *= P.step;
+        const float g = P.step * P.inv_vs;
+        gox *= g; goy *= g; goz *= g; gdx *= g; gdy *= g; gdz *= g; gtmin *= g;
     }
     if (MODE == 3 && inr) {
-        // residual of this ray; scale the unit-weight geometry sums by d(loss)/d(proj) * step
+        // residual of this ray; scale the sums by d(loss)/d(proj) * step / voxel size
         const float r = (hit ? integral * P.step : 0.f) - go;
         loss = 0.5f * P.resid_scale * r * r;
-        const float g = P.resid_scale * r * P.step;
+        const float g = P.resid_scale * r * P.step * P.inv_vs;
         gox *= g; goy *= g; goz *= g; gdx *= g; gdy *= g; gdz *= g; gtmin *= g;
     }
     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};

@@ -154,13 +154,14 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     auto use_taps = [&](const Taps& tp, float fx, float fy, float fz) {
         if (MODE == 0 || MODE >= 3) integral += tri_value(tp, tri_weights(fx, fy, fz));
         if (MODE != 0) {
-            // Sums of the voxel-unit spatial gradient; the ray's constant factor (upstream
-            // gradient x step / voxel size) is applied once at the end of the ray.
+            // the reference's per-sample order (the upstream weight gs and 1/voxel size applied
+            // at every sample; gs = 1 in the calibration modes, whose weight comes at the end)
             float sgx, sgy, sgz;
             tri_gradient(tp, fx, fy, fz, sgx, sgy, sgz);
-            gox += sgx; goy += sgy; goz += sgz;
-            gdx = fmaf(sgx, t, gdx); gdy = fmaf(sgy, t, gdy); gdz = fmaf(sgz, t, gdz);
-            gtmin = fmaf(sgz, ray.dz, fmaf(sgy, ray.dy, fmaf(sgx, ray.dx, gtmin)));
+            const float wx = sgx * inv_vs_v, wy = sgy * inv_vs_v, wz = sgz * inv_vs_v;
+            gox = fmaf(gs, wx, gox); goy = fmaf(gs, wy, goy); goz = fmaf(gs, wz, goz);
+            gdx = fmaf(gs * wx, t, gdx); gdy = fmaf(gs * wy, t, gdy); gdz = fmaf(gs * wz, t, gdz);
+            gtmin = fmaf(gs, fmaf(wz, ray.dz, fmaf(wy, ray.dy, wx * ray.dx)), gtmin);
         }
     };
     const int plane = b.H * b.W;
@@ -385,23 +386,18 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         return;
     }
     float loss = 0.f, resid = 0.f;
-    if (MODE == 1 || MODE == 2) {
-        // the ray's upstream weight and the voxel-unit scale
-        const float g = gs * P.inv_vs;
-        gox *= g; goy *= g; goz *= g; gdx *= g; gdy *= g; gdz *= g; gtmin *= g;
-    }
     if (MODE == 4 && inr) {
-        // the derivatives of this ray's projection value: step / voxel size times the sums
+        // unit-weight sums times step: the derivatives of this ray's projection value
         resid = (hit ? integral * P.step : 0.f) - go;
         loss = 0.5f * P.resid_scale * resid * resid;
-        const float g = P.step * P.inv_vs;
-        gox *= g; goy *= g; goz *= g; gdx *= g; gdy *= g; gdz *= g; gtmin *= g;
+        gox *= P.step; goy *= P.step; goz *= P.step; gdx *= P.step; gdy *= P.step; gdz *= P.step;
+        gtmin *= P.step;
     }
     if (MODE == 3 && inr) {
-        // residual of this ray; scale the sums by d(loss)/d(proj) * step / voxel size
+        // residual of this ray; scale the unit-weight geometry sums by d(loss)/d(proj) * step
         const float r = (hit ? integral * P.step : 0.f) - go;
         loss = 0.5f * P.resid_scale * r * r;
-        const float g = P.resid_scale * r * P.step * P.inv_vs;
+        const float g = P.resid_scale * r * P.step;
         gox *= g; goy *= g; goz *= g; gdx *= g; gdy *= g; gdz *= g; gtmin *= g;
     }
     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};

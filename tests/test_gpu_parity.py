@@ -329,10 +329,14 @@ def test_projector_kernel_variants_agree(tuning):
         _, gg0 = _native.fp_backward(up, vol, geom, 0.9, 0.25, 0.5, False, 0.3, -0.2, 0.1)
     torch.cuda.synchronize()
     assert torch.equal(p, ref_p)
-    # the slab kernels apply each ray's constant factor once at the end of the ray and may order
-    # the lanes differently: the geometry gradients agree with the direct kernel to FP32 rounding,
-    # and the two slab adjoints (with and without the volume gradient) agree exactly
-    assert cases.rel_l2(cases.to_np(gg), cases.to_np(ref_gg)) <= 1e-6
+    # same per-sample arithmetic as the direct kernel; another lane layout only changes the order
+    # of the per-tile wave sums
+    e_gg = cases.rel_l2(cases.to_np(gg), cases.to_np(ref_gg))
+    print(f"tuning {tuning}: geometry gradients vs direct kernel rel L2 {e_gg:.2e}")
+    if tuning[3] == 0:
+        assert torch.equal(gg, ref_gg)
+    else:
+        assert e_gg <= 1e-6
     assert torch.equal(gg0, gg)
     assert cases.rel_l2(cases.to_np(gv), cases.to_np(ref_gv)) <= 2e-6
     ref32 = oracle.fp_forward(cases.to_np(vol), [cases.to_np(g) for g in geom], 0.9, U, V, 0.25, 0.5,

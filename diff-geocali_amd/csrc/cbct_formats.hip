@@ -66,6 +66,63 @@ __global__ __launch_bounds__(256) void preprocess_kernel(const T* __restrict__ r
     }
 }
 
+// Transposing variant with paired accesses: a 64 (v) x 128 (u) tile, each lane reads two
+// consecutive u (one 4-byte uint16 pair or 8-byte float pair) and writes two consecutive v
+// (one 8-byte store).  Needs an even number of rows and columns and aligned buffers.
+constexpr int PW = 128;             // tile width along u
+constexpr int PW_PAD = PW + 1;
+
+template <typename T> struct Pair;
+template <> struct Pair<uint16_t> { typedef ushort2 type; };
+template <> struct Pair<float> { typedef float2 type; };
+
+template <typename T>
+__global__ __launch_bounds__(256) void preprocess_pair_kernel(const T* __restrict__ raw, const float* __restrict__ dark,
+                                                              const float* __restrict__ flat, int V, int U,
+                                                              float tmin, int take_log, float* __restrict__ out) {
+    __shared__ float tile[PT * PW_PAD];
+    const int tiles_u = (U + PW - 1) / PW;
+    const int tu = blockIdx.x % tiles_u, tv = blockIdx.x / tiles_u;
+    const int64_t a = blockIdx.y;
+    const int u0 = tu * PW, v0 = tv * PT;
+    const float log_tmin = __logf(tmin);
+    const T* src = raw + a * (int64_t)V * U;
+    {
+        const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+        const int u = u0 + 2 * tx;
+#pragma unroll 4
+        for (int r = ty; r < PT; r += 4) {
+            const int v = v0 + r;
+            float p0 = 0.f, p1 = 0.f;
+            if (v < V && u < U) {
+                const int64_t pix = (int64_t)v * U + u;
+                const typename Pair<T>::type x = *reinterpret_cast<const typename Pair<T>::type*>(src + pix);
+                p0 = (float)x.x;
+                p1 = (float)x.y;
+                if (take_log) {
+                    p0 = line_integral(p0, dark, flat, pix, tmin, log_tmin);
+                    p1 = line_integral(p1, dark, flat, pix + 1, tmin, log_tmin);
+                }
+            }
+            tile[r * PW_PAD + 2 * tx] = p0;
+            tile[r * PW_PAD + 2 * tx + 1] = p1;
+        }
+    }
+    __syncthreads();
+    {
+        const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;     // 32 lanes along v pairs, 8 u rows
+        const int v = v0 + 2 * tx;
+#pragma unroll 4
+        for (int r = ty; r < PW; r += 8) {
+            const int u = u0 + r;
+            if (u < U && v < V) {
+                const float2 val = make_float2(tile[(2 * tx) * PW_PAD + r], tile[(2 * tx + 1) * PW_PAD + r]);
+                *reinterpret_cast<float2*>(out + (a * U + u) * (int64_t)V + v) = val;
+            }
+        }
+    }
+}
+
 }  // namespace
 }  // namespace cbct
 
@@ -86,6 +143,19 @@ extern "C" int cbct_preprocess(const void* raw, int raw_dtype, const float* dark
     const dim3 grid((unsigned)tiles, (unsigned)A);
     hipStream_t s = (hipStream_t)stream;
     const int R = (int)rows, Cc = (int)cols, lg = take_log ? 1 : 0;
+    const size_t elem = raw_dtype == 0 ? 2 : 4;
+    if (transpose && rows % 2 == 0 && cols % 2 == 0 && ((uintptr_t)raw % (2 * elem)) == 0 &&
+        ((uintptr_t)out % 8) == 0) {
+        const dim3 pgrid((unsigned)(ceil_div(rows, (int64_t)PT) * ceil_div(cols, (int64_t)PW)), (unsigned)A);
+        if (raw_dtype == 0)
+            preprocess_pair_kernel<uint16_t><<<pgrid, 256, 0, s>>>((const uint16_t*)raw, dark, flat, R, Cc,
+                                                                  min_transmission, lg, out);
+        else
+            preprocess_pair_kernel<float><<<pgrid, 256, 0, s>>>((const float*)raw, dark, flat, R, Cc,
+                                                               min_transmission, lg, out);
+        CBCT_LAUNCHED("preprocess kernel launch");
+        return CBCT_OK;
+    }
     if (raw_dtype == 0) {
         const uint16_t* r = (const uint16_t*)raw;
         if (transpose) preprocess_kernel<uint16_t, 1><<<grid, 256, 0, s>>>(r, dark, flat, A, R, Cc, min_transmission, lg, out);

@@ -13,9 +13,12 @@ DEV = torch.device("cuda:0")
 
 @pytest.mark.parametrize("dtype", ["uint16", "float32"])
 @pytest.mark.parametrize("layout", ["AVU", "AUV"])
-def test_preprocess_matches_oracle(dtype, layout):
+@pytest.mark.parametrize("shape", [(67, 130), (66, 260)])
+def test_preprocess_matches_oracle(dtype, layout, shape):
+    """(67, 130): odd rows, the element-wise transpose; (66, 260): even sizes, the paired
+    kernel; both ragged against the tiles."""
     rng = np.random.default_rng(5)
-    A, R, C = 3, 67, 130                    # ragged against the 64x64 tiles
+    A, (R, C) = 3, shape
     dark = rng.uniform(80, 120, (R, C)).astype(np.float32)
     flat = rng.uniform(3000, 60000, (R, C)).astype(np.float32)
     p = rng.uniform(0.0, 5.0, (A, R, C))
@@ -32,8 +35,9 @@ def test_preprocess_matches_oracle(dtype, layout):
     assert np.abs(g - ref).max() <= 2e-6 * np.abs(ref).max() + 1e-6
 
 
-def test_transpose_only_is_exact():
+@pytest.mark.parametrize("shape", [(70, 129), (70, 258)])
+def test_transpose_only_is_exact(shape):
     rng = np.random.default_rng(6)
-    x = rng.standard_normal((4, 70, 129)).astype(np.float32)
+    x = rng.standard_normal((4,) + shape).astype(np.float32)
     got = formats.to_auv(torch.tensor(x, device=DEV))
     assert np.array_equal(got.cpu().numpy(), x.transpose(0, 2, 1))

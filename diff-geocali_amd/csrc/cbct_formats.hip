@@ -66,10 +66,11 @@ __global__ __launch_bounds__(256) void preprocess_kernel(const T* __restrict__ r
     }
 }
 
-// Transposing variant with paired accesses: a 64 (v) x 128 (u) tile, each lane reads two
+// Transposing variant with paired accesses on the same 64x64 tile: each lane reads two
 // consecutive u (one 4-byte uint16 pair or 8-byte float pair) and writes two consecutive v
-// (one 8-byte store).  Needs an even number of rows and columns and aligned buffers.
-constexpr int PW = 128;             // tile width along u
+// (one 8-byte store), half the memory instructions of the element-wise kernel.  Needs an even
+// number of rows and columns and aligned buffers.
+constexpr int PW = 64;              // tile width along u
 constexpr int PW_PAD = PW + 1;
 
 template <typename T> struct Pair;
@@ -88,10 +89,10 @@ __global__ __launch_bounds__(256) void preprocess_pair_kernel(const T* __restric
     const float log_tmin = __logf(tmin);
     const T* src = raw + a * (int64_t)V * U;
     {
-        const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+        const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;     // 32 lanes along u pairs, 8 v rows
         const int u = u0 + 2 * tx;
 #pragma unroll 4
-        for (int r = ty; r < PT; r += 4) {
+        for (int r = ty; r < PT; r += 8) {
             const int v = v0 + r;
             float p0 = 0.f, p1 = 0.f;
             if (v < V && u < U) {

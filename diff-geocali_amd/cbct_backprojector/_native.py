@@ -164,7 +164,7 @@ class deterministic:
 class fp_tuning:
     """Context manager that applies a projector tuning and restores the previous one."""
 
-    def __init__(self, use_slab, slab_thickness=8, lds_floats=4096, layout=0):
+    def __init__(self, use_slab, slab_thickness=8, lds_floats=4096, layout=1):
         self.new = (use_slab, slab_thickness, lds_floats, layout)
 
     def __enter__(self):

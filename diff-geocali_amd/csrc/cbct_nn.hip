@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     __shared__ __attribute__((aligned(16))) uint16_t As[GB_M * LDK];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[GB_N * LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t m0 = (int64_t)blockIdx.y * GB_M, n0 = (int64_t)blockIdx.x * GB_N;
+    const int64_t m0 = (int64_t)blockIdx.x * GB_M, n0 = (int64_t)blockIdx.y * GB_N;
     const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
     const int64_t kend = min(g.K, kbeg + g.kchunk);
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -421,7 +421,8 @@ extern "C" int cbct_nn_gemm(int trans_a, int trans_b, int64_t M, int64_t N, int6
     CBCT_CHECK_ARG((!C || ldc >= N) && (!Cb || ldcb >= N), "GEMM output leading dimension too small");
     CBCT_CHECK_ARG(act == 0 || act == 1, "act must be 0 (identity) or 1 (ReLU)");
     CBCT_CHECK_ARG(splits >= 1 && splits <= 4096, "splits must be in [1, 4096]");
-    CBCT_CHECK_ARG(ceil_div(M, (int64_t)GB_M) <= 65535, "GEMM M too large for one launch");
+    CBCT_CHECK_ARG(ceil_div(N, (int64_t)GB_N) <= 65535 && ceil_div(M, (int64_t)GB_M) <= INT32_MAX,
+                   "GEMM sizes too large for one launch");
     GemmArgs g;
     g.A = A; g.B = B; g.bias = bias; g.C = C; g.Cb = Cb; g.partial = nullptr;
     g.M = M; g.N = N; g.K = K; g.lda = lda; g.ldb = ldb; g.ldc = ldc; g.ldcb = ldcb;
@@ -437,7 +438,7 @@ extern "C" int cbct_nn_gemm(int trans_a, int trans_b, int64_t M, int64_t N, int6
         g.partial = (float*)workspace;
     }
     hipStream_t s = (hipStream_t)stream;
-    const dim3 grid((unsigned)ceil_div(N, (int64_t)GB_N), (unsigned)ceil_div(M, (int64_t)GB_M), (unsigned)g.splits);
+    const dim3 grid((unsigned)ceil_div(M, (int64_t)GB_M), (unsigned)ceil_div(N, (int64_t)GB_N), (unsigned)g.splits);
     if (!trans_a && !trans_b) gemm_kernel<0, 0><<<grid, 256, 0, s>>>(g);
     else if (!trans_a && trans_b) gemm_kernel<0, 1><<<grid, 256, 0, s>>>(g);
     else if (trans_a && !trans_b) gemm_kernel<1, 0><<<grid, 256, 0, s>>>(g);

@@ -46,7 +46,7 @@ __device__ __forceinline__ int box_index_div(unsigned i, unsigned d, unsigned ma
 // 4: Gauss-Newton step -- as 3, but each ray's eight geometry derivatives j of its projection and
 //    its residual r give the tile's normal equations: the 36 entries of sum j j^T (upper triangle,
 //    row-major), the 8 of sum j r and the loss (45 partials per tile).
-template <int MODE, bool DIV>
+template <int MODE, bool DIV, bool ZOFF = false>
 __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     extern __shared__ float lds[];
     float* vbox = lds;
@@ -140,14 +140,16 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     asm volatile("" : "+v"(inv_vs_v));
     auto voxel_coords = [&](float& x, float& y, float& z) {
         const float px = fmaf(t, ray.dx, f.ox), py = fmaf(t, ray.dy, f.oy), pz = fmaf(t, ray.dz, f.oz);
+        // ZOFF: all volume offsets are zero, and p - 0 == p exactly, so the subtraction folds away
+        const float ox_ = ZOFF ? 0.f : b.offx, oy_ = ZOFF ? 0.f : b.offy, oz_ = ZOFF ? 0.f : b.offz;
         if (MODE == 0 || MODE >= 3) {      // the forward's voxel map (division unless vs is a power of two)
-            x = to_voxel<DIV>(px, b.offx, P.vs, inv_vs_v, b.hx);
-            y = to_voxel<DIV>(py, b.offy, P.vs, inv_vs_v, b.hy);
-            z = to_voxel<DIV>(pz, b.offz, P.vs, inv_vs_v, b.hz);
+            x = to_voxel<DIV>(px, ox_, P.vs, inv_vs_v, b.hx);
+            y = to_voxel<DIV>(py, oy_, P.vs, inv_vs_v, b.hy);
+            z = to_voxel<DIV>(pz, oz_, P.vs, inv_vs_v, b.hz);
         } else {
-            x = fmaf(px - b.offx, inv_vs_v, b.hx);
-            y = fmaf(py - b.offy, inv_vs_v, b.hy);
-            z = fmaf(pz - b.offz, inv_vs_v, b.hz);
+            x = fmaf(px - ox_, inv_vs_v, b.hx);
+            y = fmaf(py - oy_, inv_vs_v, b.hy);
+            z = fmaf(pz - oz_, inv_vs_v, b.hz);
         }
     };
     // Everything a sample does once its eight taps are known.
@@ -472,13 +474,13 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     }
 }
 
-template <int MODE, bool DIV>
+template <int MODE, bool DIV, bool ZOFF>
 __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
     // Workgroups are dispatched round-robin over the 8 XCDs; permute so that consecutive tiles of a
     // view share an XCD (and its L2).
     const int xcd = blockIdx.x & 7;
     const int tile = ((P.nblocks & 7) == 0) ? xcd * (P.nblocks >> 3) + (blockIdx.x >> 3) : (int)blockIdx.x;
-    slab_tile<MODE, DIV>(P, tile);
+    slab_tile<MODE, DIV, ZOFF>(P, tile);
 }
 
 // Persistent calibration kernel: workgroups pull tiles of all problems from one counter.
@@ -520,13 +522,21 @@ int launch_fp_slab_batched(const CalibDev* probs, int nprob, int total_tiles, in
 int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream) {
     const size_t shmem = (size_t)args.cap * sizeof(float) * (mode == 2 ? 2 : 1);
     const dim3 grid((unsigned)args.nblocks), block(SL_THREADS);
+    const bool zoff = args.b.offx == 0.f && args.b.offy == 0.f && args.b.offz == 0.f;
     if (mode == 0) {
-        if (div) fp_slab_kernel<0, true><<<grid, block, shmem, stream>>>(args);
-        else fp_slab_kernel<0, false><<<grid, block, shmem, stream>>>(args);
+        if (div) {
+            if (zoff) fp_slab_kernel<0, true, true><<<grid, block, shmem, stream>>>(args);
+            else fp_slab_kernel<0, true, false><<<grid, block, shmem, stream>>>(args);
+        } else {
+            if (zoff) fp_slab_kernel<0, false, true><<<grid, block, shmem, stream>>>(args);
+            else fp_slab_kernel<0, false, false><<<grid, block, shmem, stream>>>(args);
+        }
     } else if (mode == 1) {
-        fp_slab_kernel<1, false><<<grid, block, shmem, stream>>>(args);
+        if (zoff) fp_slab_kernel<1, false, true><<<grid, block, shmem, stream>>>(args);
+        else fp_slab_kernel<1, false, false><<<grid, block, shmem, stream>>>(args);
     } else {
-        fp_slab_kernel<2, false><<<grid, block, shmem, stream>>>(args);
+        if (zoff) fp_slab_kernel<2, false, true><<<grid, block, shmem, stream>>>(args);
+        else fp_slab_kernel<2, false, false><<<grid, block, shmem, stream>>>(args);
     }
     CBCT_LAUNCHED("slab projector kernel launch");
     return CBCT_OK;

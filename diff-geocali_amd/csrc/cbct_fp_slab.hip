@@ -216,7 +216,11 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     int p = 0;
     if (fabsf(cr[1]) > fabsf(cr[p])) p = 1;
     if (fabsf(cr[2]) > fabsf(cr[p])) p = 2;
-    const float dirf = cr[p] > 0.f ? 1.f : -1.f;
+    // every lane computed the same axis and direction; make that explicit (scalar registers,
+    // uniform branches in the march loop)
+    p = __builtin_amdgcn_readfirstlane(p);
+    const int fwd = __builtin_amdgcn_readfirstlane(cr[p] > 0.f ? 1 : 0);
+    const float dirf = fwd ? 1.f : -1.f;
     float corner[4][3];
     pixel_dir(f, (float)u_lo, (float)v_lo, P.pix, corner[0]);
     pixel_dir(f, (float)u_hi, (float)v_lo, P.pix, corner[1]);
@@ -289,6 +293,13 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 if (dirf > 0.f) { s_lo = next; s_hi = next + thick - 1; }
                 else { s_hi = next; s_lo = next - thick + 1; }
                 nonempty = slab_box(s_lo, s_hi, lo, hi);
+                // identical in every lane: keep the box in scalar registers
+                nonempty = __builtin_amdgcn_readfirstlane(nonempty ? 1 : 0) != 0;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    lo[i] = __builtin_amdgcn_readfirstlane(lo[i]);
+                    hi[i] = __builtin_amdgcn_readfirstlane(hi[i]);
+                }
                 if (!nonempty || thick == 1 ||
                     (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) <= P.cap)
                     break;
@@ -324,11 +335,12 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
                 const int xr = (int)flx, yr = (int)fly, zr = (int)flz;
                 const int fp = p == 0 ? xr : (p == 1 ? yr : zr);
-                if (dirf > 0.f ? fp > s_hi : fp < s_lo) break;
+                if (fwd ? fp > s_hi : fp < s_lo) break;
                 const unsigned ox = (unsigned)(xr - lo[0]), oy = (unsigned)(yr - lo[1]), oz = (unsigned)(zr - lo[2]);
                 if (ox < bx1 && oy < by1 && oz < bz1) {
                     const float fx = x - flx, fy = y - fly, fz = z - flz;
-                    const int idx = __mul24(__mul24((int)oz, B1) + (int)oy, B0) + (int)ox;
+                    // 24-bit unsigned products (box offsets below the LDS capacity): full-rate multiplies
+                    const int idx = (int)(__umul24(oz, (unsigned)PL) + __umul24(oy, (unsigned)B0) + ox);
                     const Taps tp = load_taps_strided(vbox, idx, 1, B0, PL);
                     if (MODE == 2) {
                         // scaled trilinear weights, the scale folded into the z factors

@@ -96,3 +96,28 @@ def test_geometry_from_table_rejects_missing_or_ragged_columns():
         formats.geometry_from_table({"DD": [1.0]}, 4, 4, 1.0)
     with pytest.raises(ValueError, match="one value per view"):
         formats.geometry_from_table({"SAD_Rs": [1.0, 2.0], "DD": [3.0]}, 4, 4, 1.0)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_autotune_cache_round_trip_and_shape_classes(tmp_path, monkeypatch):
+    from cbct_backprojector import autotune
+    monkeypatch.setenv("CBCT_AUTOTUNE_CACHE", str(tmp_path / "sub" / "cache.json"))
+    k1 = autotune.shape_key("adjoint", (256, 256, 256), 768, 640, 0.25, 0.5, 1100.0, 600.0)
+    k2 = autotune.shape_key("adjoint", (250, 256, 256), 760, 640, 0.25, 0.5, 1100.0, 600.0)
+    k3 = autotune.shape_key("forward", (256, 256, 256), 768, 640, 0.25, 0.5, 1100.0, 600.0)
+    assert k1 == k2 and k1 != k3                       # sizes round to multiples of 16
+    autotune._store(k1, {"best": [12, 4096, 1], "ms_per_view": 0.3})
+    autotune._store(k3, {"best": [8, 8192, 0], "ms_per_view": 0.2})
+    data = autotune._load()
+    assert data[k1]["best"] == [12, 4096, 1] and data[k3]["best"] == [8, 8192, 0]
+    assert len(autotune.candidates()) == 24 and autotune.DEFAULT in autotune.candidates()

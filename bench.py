@@ -489,7 +489,7 @@ def preprocess_bench(args, torch, device):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the operators run on a single device; multi-GPU sharding is design-only).", file=sys.stderr)
         sys.exit(2)
     import torch

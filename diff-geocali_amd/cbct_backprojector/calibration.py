@@ -14,7 +14,7 @@ projection, residual and geometry gradient, a fixed-order per-view reduction,
 the Adam update), and all steps can be captured in one hipGraph
 (``cbct_calib_run``).
 
-``reprojection_error_px`` is the accuracy measure used to judge a fit: the RMS
+``reprojection_error_px`` is the accuracy measure used to assess a fit: the RMS
 over views and nine fixed points (the volume's corners and centre) of the
 detector-position difference, in pixels, between two geometries, with the
 point-to-detector map of the backprojector

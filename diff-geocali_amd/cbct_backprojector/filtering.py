@@ -88,7 +88,7 @@ def fdk_filter(projections, pixel_size, sod, sid, alpha=0.8, *, det_u0=None, det
     sod = _filter_distance("sod", sod, projections)
     sid = _filter_distance("sid", sid, projections)
     if projections.device.type != "cuda":
-        raise ValueError("fdk_filter runs on the GPU in this build; move projections to a GPU device")
+        raise ValueError("fdk_filter runs on the GPU only; move projections to a GPU device")
     if (det_u0 is None) != (det_v0 is None):
         raise ValueError("det_u0 and det_v0 must be given together")
     extra = []

@@ -276,7 +276,40 @@ def calibration_bench(args, torch, device):
         "roofline": {"bound": "hbm", "kernel": "fused calibration march (whole graph timed)", "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
     }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_calibration(probs, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_calibration(probs, budget_s):
+    """One calibration march restated on the CPU: the FP32 C oracle's forward projection and
+    geometry-only adjoint (residual as upstream weight) of single views of the problems, in
+    turn, until the budget is used."""
+    import numpy as np
+
+    import oracle
+    threads = oracle.max_threads()
+    done_updates, t_total, views = 0, 0.0, 0
+    for k, p in enumerate(probs):
+        A, U, V = p.measured.shape
+        a = (7 * k) % A
+        g1 = [np.ascontiguousarray(p.geometry[i, a:a + 1].cpu().numpy()) for i in range(8)]
+        vol = p.volume.cpu().numpy()
+        meas = p.measured[a:a + 1].cpu().numpy()
+        t0 = time.perf_counter()
+        proj = oracle.fp_forward(vol, g1, p.pixel_size, U, V, p.step_size, p.voxel_size, mode="f32",
+                                 threads=threads)
+        up = (2.0 / (A * U * V)) * (proj - meas)
+        oracle.fp_backward(up.astype(np.float32), vol, g1, p.pixel_size, p.step_size, p.voxel_size,
+                           need_volume=False, mode="f32", threads=threads)
+        t_total += time.perf_counter() - t0
+        done_updates += vol.size
+        views += 1
+        if t_total >= budget_s:
+            break
+    return {"value": done_updates / t_total / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
+            "sample": f"one view of each of {views} problems: forward projection + geometry-only adjoint "
+                      f"(FP32 C oracle), {t_total:.1f} s"}
 
 
 def warm_start_bench(args, torch, device):

@@ -1,8 +1,11 @@
 // Shared device helpers for the cone-beam CT kernels (gfx950).
 //
 // All files are compiled with -ffp-contract=off: every fused multiply-add is
-// written out as fmaf(), so the float operation order below is the stated
-// FP32 reference schedule and is reproduced bit for bit by the CPU oracle.
+// written out as fmaf(), so the float operation order below is fixed.  Ray
+// set-up, sample positions, cells and fractions follow the FP32 reference
+// schedule bit for bit; the trilinear interpolant is evaluated in a factored
+// (nested-lerp) form that differs from the reference's weighted sum by FP32
+// rounding only.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -153,6 +156,17 @@ __device__ __forceinline__ float to_voxel(float p, float off, float vs, float in
     return fmaf(p - off, inv_vs, half);
 }
 
+// floor(x) as an int in one instruction (v_cvt_flr_i32_f32) and x - floor(x)
+// (v_fract_f32; identical to the subtraction for every coordinate except
+// -2^-25 < x < 0, where it gives the largest float below 1 instead of 1 -- a
+// cell whose two x corners are the same clamped voxel, so nothing changes).
+__device__ __forceinline__ int floor_int(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+__device__ __forceinline__ float frac(float x) { return __builtin_amdgcn_fractf(x); }
+
 // Trilinear corner set of a sample: clamp-to-edge corner offsets and raw
 // fractions (csrc/src_cbct_forwardprojector_cuda.cu:186-229).
 struct Corners {
@@ -161,9 +175,9 @@ struct Corners {
 };
 
 __device__ __forceinline__ Corners corners(const Box& b, float x, float y, float z) {
-    const int xr = (int)floorf(x), yr = (int)floorf(y), zr = (int)floorf(z);
+    const int xr = floor_int(x), yr = floor_int(y), zr = floor_int(z);
     Corners c;
-    c.fx = x - (float)xr; c.fy = y - (float)yr; c.fz = z - (float)zr;
+    c.fx = frac(x); c.fy = frac(y); c.fz = frac(z);
     const int x0 = max(0, min(xr, b.W - 1)), x1 = max(0, min(xr + 1, b.W - 1));
     const int y0 = max(0, min(yr, b.H - 1)), y1 = max(0, min(yr + 1, b.H - 1));
     const int z0 = max(0, min(zr, b.D - 1)), z1 = max(0, min(zr + 1, b.D - 1));
@@ -183,9 +197,9 @@ struct Cell {
 };
 
 __device__ __forceinline__ Cell cell_of(const Box& b, float x, float y, float z) {
-    const int xr = (int)floorf(x), yr = (int)floorf(y), zr = (int)floorf(z);
+    const int xr = floor_int(x), yr = floor_int(y), zr = floor_int(z);
     Cell c;
-    c.fx = x - (float)xr; c.fy = y - (float)yr; c.fz = z - (float)zr;
+    c.fx = frac(x); c.fy = frac(y); c.fz = frac(z);
     c.x0 = max(0, min(xr, b.W - 1));
     c.y0 = max(0, min(yr, b.H - 1));
     c.z0 = max(0, min(zr, b.D - 1));
@@ -229,25 +243,31 @@ __device__ __forceinline__ Weights tri_weights(float fx, float fy, float fz) {
     return w;
 }
 
-__device__ __forceinline__ float tri_value(const Taps& t, const Weights& w) {
-    float acc = t.v000 * w.w000;
-    acc = fmaf(t.v100, w.w100, acc); acc = fmaf(t.v010, w.w010, acc); acc = fmaf(t.v110, w.w110, acc);
-    acc = fmaf(t.v001, w.w001, acc); acc = fmaf(t.v101, w.w101, acc); acc = fmaf(t.v011, w.w011, acc);
-    acc = fmaf(t.v111, w.w111, acc);
-    return acc;
+// Trilinear value and spatial gradient (voxel units) of the eight taps, as
+// nested linear interpolations along x, then y, then z.  This is the
+// reference's interpolant (csrc/src_cbct_forwardprojector_cuda.cu:186-229) and
+// its analytic derivative (:242-250) in factored form: 14 operations for the
+// value and 8 more for the gradient (the weighted corner sum takes 23 and 36);
+// the results differ from the weighted sum by FP32 rounding only.
+__device__ __forceinline__ void tri_eval(const Taps& t, float fx, float fy, float fz,
+                                         float& val, float& gx, float& gy, float& gz) {
+    const float d00 = t.v100 - t.v000, d10 = t.v110 - t.v010;      // x differences per (y, z)
+    const float d01 = t.v101 - t.v001, d11 = t.v111 - t.v011;
+    const float c00 = fmaf(fx, d00, t.v000), c10 = fmaf(fx, d10, t.v010);
+    const float c01 = fmaf(fx, d01, t.v001), c11 = fmaf(fx, d11, t.v011);
+    const float e0d = c10 - c00, e1d = c11 - c01;                    // y differences per z
+    const float e0 = fmaf(fy, e0d, c00), e1 = fmaf(fy, e1d, c01);
+    gz = e1 - e0;
+    val = fmaf(fz, gz, e0);
+    gy = fmaf(fz, e1d - e0d, e0d);
+    const float x0 = fmaf(fy, d10 - d00, d00), x1 = fmaf(fy, d11 - d01, d01);
+    gx = fmaf(fz, x1 - x0, x0);
 }
 
-// Analytic spatial gradient of the trilinear interpolant in voxel units
-// (csrc/src_cbct_forwardprojector_cuda.cu:242-250).
-__device__ __forceinline__ void tri_gradient(const Taps& t, float fx, float fy, float fz,
-                                             float& gx, float& gy, float& gz) {
-    const float mx = 1.0f - fx, my = 1.0f - fy, mz = 1.0f - fz;
-    gx = fmaf(fz, fmaf(fy, t.v111 - t.v011, my * (t.v101 - t.v001)),
-              mz * fmaf(fy, t.v110 - t.v010, my * (t.v100 - t.v000)));
-    gy = fmaf(fz, fmaf(fx, t.v111 - t.v101, mx * (t.v011 - t.v001)),
-              mz * fmaf(fx, t.v110 - t.v100, mx * (t.v010 - t.v000)));
-    gz = fmaf(fy, fmaf(fx, t.v111 - t.v110, mx * (t.v011 - t.v010)),
-              my * fmaf(fx, t.v101 - t.v100, mx * (t.v001 - t.v000)));
+__device__ __forceinline__ float tri_value(const Taps& t, float fx, float fy, float fz) {
+    float val, gx, gy, gz;
+    tri_eval(t, fx, fy, fz, val, gx, gy, gz);
+    return val;
 }
 
 // Wave-level (64 lanes) sum.

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(FP_THREADS) void fp_forward_kernel(
                 const float z = to_voxel<DIV>(pz, b.offz, vs, inv_vs, b.hz);
                 const Corners c = corners(b, x, y, z);
                 const Taps tp = load_taps(vol, c);
-                integral += tri_value(tp, tri_weights(c.fx, c.fy, c.fz));
+                integral += tri_value(tp, c.fx, c.fy, c.fz);
             }
             integral *= step;
         }
@@ -115,8 +115,8 @@ __global__ __launch_bounds__(FP_THREADS) void fp_backward_kernel(
                     atomicAdd(p + c.dz1 + c.dy1, gs * w.w011);
                     atomicAdd(p + c.dz1 + c.dy1 + c.dx1, gs * w.w111);
                 }
-                float sgx, sgy, sgz;
-                tri_gradient(tp, c.fx, c.fy, c.fz, sgx, sgy, sgz);
+                float sval, sgx, sgy, sgz;
+                tri_eval(tp, c.fx, c.fy, c.fz, sval, sgx, sgy, sgz);
                 const float wx = sgx * inv_vs, wy = sgy * inv_vs, wz = sgz * inv_vs;
                 gox = fmaf(gs, wx, gox); goy = fmaf(gs, wy, goy); goz = fmaf(gs, wz, goz);
                 gdx = fmaf(gs * wx, t, gdx); gdy = fmaf(gs * wy, t, gdy); gdz = fmaf(gs * wz, t, gdz);

@@ -14,7 +14,7 @@
 //      the eight weighted contributions to the LDS gradient box);
 //   4. flushes the gradient box to HBM with one atomic add per non-zero voxel.
 // The arithmetic of every sample is the same float code as the direct kernels
-// (same t schedule, same cells, same weights), so projections and geometry
+// (same t schedule, same cells, same interpolation), so projections and geometry
 // gradients are bit-identical to them; only the order of the volume-gradient
 // atomics differs.  A sample whose taps fall outside the staged box (never in
 // a regular cone-beam geometry) and tiles whose rays do not share one principal
@@ -23,6 +23,7 @@
 #include "cbct_fp_slab.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace cbct {
 
@@ -76,6 +77,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         if ((MODE == 0 || MODE >= 3 || go != 0.f) && make_ray(f, u, v, P.pix, ray)) {
             valid = true;
             if (intersect(b, f, ray, tmin, tmax, entry)) { hit = true; alive = true; t = tmin; }
+            else tmax = -1.f;        // keep t <= tmax equivalent to alive (t = 0) for a miss
         }
     }
     const float gs = MODE >= 3 ? 1.0f : go * P.step;
@@ -152,23 +154,26 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             z = fmaf(pz - oz_, inv_vs_v, b.hz);
         }
     };
-    // Everything a sample does once its eight taps are known.
-    auto use_taps = [&](const Taps& tp, float fx, float fy, float fz) {
-        if (MODE == 0 || MODE >= 3) integral += tri_value(tp, tri_weights(fx, fy, fz));
+    // Everything a sample does once its eight taps are known.  Lanes with go == false (no sample in
+    // the current slab) leave every sum unchanged.
+    auto use_taps = [&](const Taps& tp, float fx, float fy, float fz, bool go) {
+        float val, sgx, sgy, sgz;
+        tri_eval(tp, fx, fy, fz, val, sgx, sgy, sgz);
+        if (MODE == 0 || MODE >= 3) integral += go ? val : 0.f;
         if (MODE != 0) {
             // the reference's per-sample order (the upstream weight gs and 1/voxel size applied
             // at every sample; gs = 1 in the calibration modes, whose weight comes at the end)
-            float sgx, sgy, sgz;
-            tri_gradient(tp, fx, fy, fz, sgx, sgy, sgz);
+            const float g = go ? gs : 0.f;
             const float wx = sgx * inv_vs_v, wy = sgy * inv_vs_v, wz = sgz * inv_vs_v;
-            gox = fmaf(gs, wx, gox); goy = fmaf(gs, wy, goy); goz = fmaf(gs, wz, goz);
-            gdx = fmaf(gs * wx, t, gdx); gdy = fmaf(gs * wy, t, gdy); gdz = fmaf(gs * wz, t, gdz);
-            gtmin = fmaf(gs, fmaf(wz, ray.dz, fmaf(wy, ray.dy, wx * ray.dx)), gtmin);
+            gox = fmaf(g, wx, gox); goy = fmaf(g, wy, goy); goz = fmaf(g, wz, goz);
+            gdx = fmaf(g * wx, t, gdx); gdy = fmaf(g * wy, t, gdy); gdz = fmaf(g * wz, t, gdz);
+            gtmin = fmaf(g, fmaf(wz, ray.dz, fmaf(wy, ray.dy, wx * ray.dx)), gtmin);
         }
     };
     const int plane = b.H * b.W;
-    // A sample from global memory with the reference's clamp-to-edge corners.
-    auto global_sample = [&](float x, float y, float z) {
+    // The taps of a sample from global memory with the reference's clamp-to-edge corners (and, for
+    // the adjoint, its eight volume-gradient contributions).
+    auto global_taps = [&](float x, float y, float z) {
         const Cell c = cell_of(b, x, y, z);
         const int i000 = (c.z0 * b.H + c.y0) * b.W + c.x0;
         const int ox1 = c.sx, oy1 = c.sy * b.W, oz1 = c.sz * plane;
@@ -200,12 +205,13 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             atomicAdd(q + oz1 + oy1, gs * w.w011);
             atomicAdd(q + oz1 + oy1 + ox1, gs * w.w111);
         }
-        use_taps(tp, c.fx, c.fy, c.fz);
+        return tp;
     };
     auto global_step = [&]() {
         float x, y, z;
         voxel_coords(x, y, z);
-        global_sample(x, y, z);
+        const Taps tp = global_taps(x, y, z);
+        use_taps(tp, frac(x), frac(y), frac(z), true);
     };
 
     // ---- principal axis of the tile (uniform over the workgroup)
@@ -280,6 +286,71 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         return front && lo[0] < hi[0] && lo[1] < hi[1] && lo[2] < hi[2];
     };
 
+    // The samples of one slab: those whose principal floor coordinate lies in [s_lo, s_hi], with
+    // the taps read from the staged box.  PAX / FWD (the tile's principal axis and direction) are
+    // compile-time constants, so the slab-exit test is one compare.  The loop runs while any lane
+    // of the wave has a sample in the slab; a lane without one rides along with its updates masked
+    // (go == false), so the loop has no divergent exits.  A sample whose taps leave the box (not
+    // seen in regular cone-beam geometry; every sample when the slab is not staged) reads them
+    // from global memory.
+    auto march = [&](auto pax_c, auto fwd_c, int s_lo, int s_hi, const int* lo, int B0, int B1, int B2,
+                     bool staged) {
+        constexpr int PAX = decltype(pax_c)::value;
+        constexpr bool FWD = decltype(fwd_c)::value;
+        const unsigned bx1 = staged ? (unsigned)(B0 - 1) : 0u, by1 = staged ? (unsigned)(B1 - 1) : 0u;
+        const unsigned bz1 = staged ? (unsigned)(B2 - 1) : 0u;
+        const int PL = B0 * B1;
+        // largest base offset whose eight taps stay inside the staged box (that of the last cell)
+        const unsigned safe = staged ? (unsigned)(B0 * B1 * B2 - PL - B0 - 2) : 0u;
+        for (;;) {
+            float x, y, z;
+            voxel_coords(x, y, z);
+            const int xr = floor_int(x), yr = floor_int(y), zr = floor_int(z);
+            const int fp = PAX == 0 ? xr : (PAX == 1 ? yr : zr);
+            // A lane is alive while t <= tmax (never, for a ray that misses the volume).  Lane
+            // predicates are ands of compares; the loop exit is a ballot of the same compares
+            // (scalar registers only).
+            const bool live = t <= tmax;
+            const bool ins = FWD ? fp <= s_hi : fp >= s_lo;
+            const uint64_t m_go = __builtin_amdgcn_ballot_w64(live) & __builtin_amdgcn_ballot_w64(ins);
+            if (m_go == 0ull) break;
+            const unsigned ox = (unsigned)(xr - lo[0]), oy = (unsigned)(yr - lo[1]), oz = (unsigned)(zr - lo[2]);
+            const bool go = live && ins;
+            const bool gl = go && ox < bx1 && oy < by1 && oz < bz1;
+            const float fx = frac(x), fy = frac(y), fz = frac(z);
+            if (staged) {
+                // 24-bit unsigned products (box offsets below the LDS capacity): full-rate multiplies.
+                // Lanes outside the box read a clamped in-box address and discard the result.
+                const unsigned raw = __umul24(oz, (unsigned)PL) + __umul24(oy, (unsigned)B0) + ox;
+                const int idx = (int)min(raw, safe);
+                const Taps tp = load_taps_strided(vbox, idx, 1, B0, PL);
+                use_taps(tp, fx, fy, fz, gl);
+                if (MODE == 2 && gl) {
+                    // after the taps are consumed: the scaled trilinear weights as a z, then y,
+                    // then x split of gsq (each pair sums to its parent)
+                    const float g1 = gsq * fz, g0 = gsq - g1;
+                    const float g01 = g0 * fy, g00 = g0 - g01, g11 = g1 * fy, g10 = g1 - g11;
+                    const float a00 = g00 * fx, a01 = g01 * fx, a10 = g10 * fx, a11 = g11 * fx;
+                    unsigned* qi = gboxu + idx;
+                    atomicAdd(qi, (unsigned)__float2int_rn(g00 - a00));
+                    atomicAdd(qi + 1, (unsigned)__float2int_rn(a00));
+                    atomicAdd(qi + B0, (unsigned)__float2int_rn(g01 - a01));
+                    atomicAdd(qi + B0 + 1, (unsigned)__float2int_rn(a01));
+                    atomicAdd(qi + PL, (unsigned)__float2int_rn(g10 - a10));
+                    atomicAdd(qi + PL + 1, (unsigned)__float2int_rn(a10));
+                    atomicAdd(qi + PL + B0, (unsigned)__float2int_rn(g11 - a11));
+                    atomicAdd(qi + PL + B0 + 1, (unsigned)__float2int_rn(a11));
+                }
+            }
+            if (go && !gl) {      // a divergent branch that no lane takes costs a skip
+                const Taps tg = global_taps(x, y, z);
+                use_taps(tg, fx, fy, fz, true);
+            }
+            t = go ? t + P.step : t;
+        }
+        alive = t <= tmax;
+    };
+
     if (slab_ok) {
         const int Np = Nax[p];
         // Slabs cover the principal floor coordinates -1 .. Np-1 in the rays' direction; a slab is
@@ -325,44 +396,14 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 }
                 __syncthreads();
             }
-            // advance the lane's ray through this slab
-            const unsigned bx1 = staged ? (unsigned)(B0 - 1) : 0u, by1 = staged ? (unsigned)(B1 - 1) : 0u;
-            const unsigned bz1 = staged ? (unsigned)(B2 - 1) : 0u;
-            const int PL = B0 * B1;
-            while (in_slabs && alive) {
-                float x, y, z;
-                voxel_coords(x, y, z);
-                const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
-                const int xr = (int)flx, yr = (int)fly, zr = (int)flz;
-                const int fp = p == 0 ? xr : (p == 1 ? yr : zr);
-                if (fwd ? fp > s_hi : fp < s_lo) break;
-                const unsigned ox = (unsigned)(xr - lo[0]), oy = (unsigned)(yr - lo[1]), oz = (unsigned)(zr - lo[2]);
-                if (ox < bx1 && oy < by1 && oz < bz1) {
-                    const float fx = x - flx, fy = y - fly, fz = z - flz;
-                    // 24-bit unsigned products (box offsets below the LDS capacity): full-rate multiplies
-                    const int idx = (int)(__umul24(oz, (unsigned)PL) + __umul24(oy, (unsigned)B0) + ox);
-                    const Taps tp = load_taps_strided(vbox, idx, 1, B0, PL);
-                    if (MODE == 2) {
-                        // scaled trilinear weights, the scale folded into the z factors
-                        const float mx = 1.0f - fx, my = 1.0f - fy;
-                        const float g0 = gsq * (1.0f - fz), g1 = gsq * fz;
-                        const float g00 = g0 * my, g01 = g0 * fy, g10 = g1 * my, g11 = g1 * fy;
-                        unsigned* qi = gboxu + idx;
-                        atomicAdd(qi, (unsigned)__float2int_rn(g00 * mx));
-                        atomicAdd(qi + 1, (unsigned)__float2int_rn(g00 * fx));
-                        atomicAdd(qi + B0, (unsigned)__float2int_rn(g01 * mx));
-                        atomicAdd(qi + B0 + 1, (unsigned)__float2int_rn(g01 * fx));
-                        atomicAdd(qi + PL, (unsigned)__float2int_rn(g10 * mx));
-                        atomicAdd(qi + PL + 1, (unsigned)__float2int_rn(g10 * fx));
-                        atomicAdd(qi + PL + B0, (unsigned)__float2int_rn(g11 * mx));
-                        atomicAdd(qi + PL + B0 + 1, (unsigned)__float2int_rn(g11 * fx));
-                    }
-                    use_taps(tp, fx, fy, fz);
-                } else {
-                    global_sample(x, y, z);
-                }
-                t += P.step;
-                alive = t <= tmax;
+            // advance the lane's ray through this slab (principal axis and direction as constants)
+            switch (p * 2 + fwd) {
+                case 0: march(std::integral_constant<int, 0>(), std::false_type(), s_lo, s_hi, lo, B0, B1, B2, staged); break;
+                case 1: march(std::integral_constant<int, 0>(), std::true_type(), s_lo, s_hi, lo, B0, B1, B2, staged); break;
+                case 2: march(std::integral_constant<int, 1>(), std::false_type(), s_lo, s_hi, lo, B0, B1, B2, staged); break;
+                case 3: march(std::integral_constant<int, 1>(), std::true_type(), s_lo, s_hi, lo, B0, B1, B2, staged); break;
+                case 4: march(std::integral_constant<int, 2>(), std::false_type(), s_lo, s_hi, lo, B0, B1, B2, staged); break;
+                default: march(std::integral_constant<int, 2>(), std::true_type(), s_lo, s_hi, lo, B0, B1, B2, staged); break;
             }
             if (MODE == 2 && staged) {
                 __syncthreads();

@@ -156,15 +156,11 @@ __device__ __forceinline__ float to_voxel(float p, float off, float vs, float in
     return fmaf(p - off, inv_vs, half);
 }
 
-// floor(x) as an int in one instruction (v_cvt_flr_i32_f32) and x - floor(x)
-// (v_fract_f32; identical to the subtraction for every coordinate except
-// -2^-25 < x < 0, where it gives the largest float below 1 instead of 1 -- a
-// cell whose two x corners are the same clamped voxel, so nothing changes).
-__device__ __forceinline__ int floor_int(float x) {
-    int r;
-    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
-    return r;
-}
+// floor(x) as an int, and x - floor(x) in one instruction (v_fract_f32; identical
+// to the subtraction for every coordinate except -2^-25 < x < 0, where it gives
+// the largest float below 1 instead of 1 -- a cell whose two x corners are the
+// same clamped voxel, so nothing changes).
+__device__ __forceinline__ int floor_int(float x) { return (int)floorf(x); }
 __device__ __forceinline__ float frac(float x) { return __builtin_amdgcn_fractf(x); }
 
 // Trilinear corner set of a sample: clamp-to-edge corner offsets and raw

@@ -117,11 +117,13 @@ __global__ __launch_bounds__(FP_THREADS) void fp_backward_kernel(
                 }
                 float sval, sgx, sgy, sgz;
                 tri_eval(tp, c.fx, c.fy, c.fz, sval, sgx, sgy, sgz);
-                const float wx = sgx * inv_vs, wy = sgy * inv_vs, wz = sgz * inv_vs;
-                gox = fmaf(gs, wx, gox); goy = fmaf(gs, wy, goy); goz = fmaf(gs, wz, goz);
-                gdx = fmaf(gs * wx, t, gdx); gdy = fmaf(gs * wy, t, gdy); gdz = fmaf(gs * wz, t, gdz);
-                gtmin = fmaf(gs, fmaf(wz, ray.dz, fmaf(wy, ray.dy, wx * ray.dx)), gtmin);
+                // unscaled sums; the constant factor gs / voxel size is applied once below
+                gox += sgx; goy += sgy; goz += sgz;
+                gdx = fmaf(sgx, t, gdx); gdy = fmaf(sgy, t, gdy); gdz = fmaf(sgz, t, gdz);
             }
+            const float sc = gs * inv_vs;
+            gtmin = sc * fmaf(ray.dz, goz, fmaf(ray.dy, goy, ray.dx * gox));
+            gox *= sc; goy *= sc; goz *= sc; gdx *= sc; gdy *= sc; gdz *= sc;
             // The samples are anchored at the entry face (:454-467).
             if (entry == 0) { gox -= gtmin / ray.dx; gdx -= gtmin * tmin / ray.dx; }
             else if (entry == 1) { goy -= gtmin / ray.dy; gdy -= gtmin * tmin / ray.dy; }

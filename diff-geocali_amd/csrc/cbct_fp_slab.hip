@@ -34,11 +34,6 @@ __device__ __forceinline__ void pixel_dir(const ViewFrame& f, float u, float v, 
     r[2] = vp - f.oz;
 }
 
-// i / d for i < 2^16 and 1 <= d < 2^16, with magic = 0xffffffff / d + 1 (which wraps to 0 for d == 1).
-__device__ __forceinline__ int box_index_div(unsigned i, unsigned d, unsigned magic) {
-    return d == 1u ? (int)i : (int)__umulhi(i, magic);
-}
-
 // One 16x16 tile of rays of one problem.
 // MODE 0: forward; 1: adjoint, geometry gradients only; 2: adjoint with the volume gradient;
 // 3: calibration step -- forward, squared residual against the measured projections (P.grad) and
@@ -47,11 +42,13 @@ __device__ __forceinline__ int box_index_div(unsigned i, unsigned d, unsigned ma
 // 4: Gauss-Newton step -- as 3, but each ray's eight geometry derivatives j of its projection and
 //    its residual r give the tile's normal equations: the 36 entries of sum j j^T (upper triangle,
 //    row-major), the 8 of sum j r and the loss (45 partials per tile).
-template <int MODE, bool DIV, bool ZOFF = false>
+template <int MODE, bool DIV, bool ZOFF = false, int CAPC = 0>
 __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     extern __shared__ float lds[];
     float* vbox = lds;
-    float* gbox = lds + P.cap;
+    // CAPC > 0: the LDS capacity as a compile-time constant (== P.cap), so the gradient box's
+    // addresses are the tap addresses plus an immediate offset
+    float* gbox = lds + (CAPC > 0 ? CAPC : P.cap);
     const Box& b = P.b;
 
     // ---- tile and ray of this lane
@@ -161,13 +158,12 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         tri_eval(tp, fx, fy, fz, val, sgx, sgy, sgz);
         if (MODE == 0 || MODE >= 3) integral += go ? val : 0.f;
         if (MODE != 0) {
-            // the reference's per-sample order (the upstream weight gs and 1/voxel size applied
-            // at every sample; gs = 1 in the calibration modes, whose weight comes at the end)
-            const float g = go ? gs : 0.f;
-            const float wx = sgx * inv_vs_v, wy = sgy * inv_vs_v, wz = sgz * inv_vs_v;
-            gox = fmaf(g, wx, gox); goy = fmaf(g, wy, goy); goz = fmaf(g, wz, goz);
-            gdx = fmaf(g * wx, t, gdx); gdy = fmaf(g * wy, t, gdy); gdz = fmaf(g * wz, t, gdz);
-            gtmin = fmaf(g, fmaf(wz, ray.dz, fmaf(wy, ray.dy, wx * ray.dx)), gtmin);
+            // unscaled sums of the voxel-unit spatial gradient and of its product with t; the
+            // ray's constant factors (upstream weight, step, 1/voxel size) are applied once after
+            // the march, and the t_min sum is their dot product with the direction
+            sgx = go ? sgx : 0.f; sgy = go ? sgy : 0.f; sgz = go ? sgz : 0.f;
+            gox += sgx; goy += sgy; goz += sgz;
+            gdx = fmaf(sgx, t, gdx); gdy = fmaf(sgy, t, gdy); gdz = fmaf(sgz, t, gdz);
         }
     };
     const int plane = b.H * b.W;
@@ -262,20 +258,24 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     // q = s_hi + 1, widened by the +1 taps and a rounding margin, limited to [-1, N].  The staged
     // copy holds vol[clamp(.)] at every raw coordinate, so taps read from it need no clamping.
     // Returns false when no cell fits in it.
+    // Lateral slopes of the four corner rays (voxel units per principal voxel), once per tile:
+    // between the planes q = s_lo and q = s_hi + 1 a corner ray's lateral coordinates are
+    // xs + (q - xs_p) * slope.  Every corner ray moves in the march direction (slab_ok), so the
+    // planes lie in front of the source exactly when (q - xs_p) has the direction's sign.
+    float sl1[4], sl2[4];
+    for (int j = 0; j < 4; ++j) {
+        sl1[j] = corner[j][o1] / corner[j][p];
+        sl2[j] = corner[j][o2] / corner[j][p];
+    }
     auto slab_box = [&](int s_lo, int s_hi, int lo[3], int hi[3]) {
+        const float q0 = (float)s_lo - xs[p], q1 = (float)(s_hi + 1) - xs[p];
+        const bool front = q0 * dirf > 0.f && q1 * dirf > 0.f;
         float mn1 = FLT_MAX, mx1 = -FLT_MAX, mn2 = FLT_MAX, mx2 = -FLT_MAX;
-        bool front = true;
         for (int j = 0; j < 4; ++j) {
-            const float rp = corner[j][p] * P.inv_vs;
-            for (int e = 0; e < 2; ++e) {
-                const float q = (float)(e == 0 ? s_lo : s_hi + 1);
-                const float lam = (q - xs[p]) / rp;
-                front = front && lam > 0.f;
-                const float c1 = fmaf(lam, corner[j][o1] * P.inv_vs, xs[o1]);
-                const float c2 = fmaf(lam, corner[j][o2] * P.inv_vs, xs[o2]);
-                mn1 = fminf(mn1, c1); mx1 = fmaxf(mx1, c1);
-                mn2 = fminf(mn2, c2); mx2 = fmaxf(mx2, c2);
-            }
+            const float a1 = fmaf(q0, sl1[j], xs[o1]), b1 = fmaf(q1, sl1[j], xs[o1]);
+            const float a2 = fmaf(q0, sl2[j], xs[o2]), b2 = fmaf(q1, sl2[j], xs[o2]);
+            mn1 = fminf(mn1, fminf(a1, b1)); mx1 = fmaxf(mx1, fmaxf(a1, b1));
+            mn2 = fminf(mn2, fminf(a2, b2)); mx2 = fmaxf(mx2, fmaxf(a2, b2));
         }
         lo[p] = max(-1, s_lo);
         hi[p] = min(Nax[p], s_hi + 1);
@@ -326,8 +326,8 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 const Taps tp = load_taps_strided(vbox, idx, 1, B0, PL);
                 use_taps(tp, fx, fy, fz, gl);
                 if (MODE == 2 && gl) {
-                    // after the taps are consumed: the scaled trilinear weights as a z, then y,
-                    // then x split of gsq (each pair sums to its parent)
+                    // the scaled trilinear weights as a z, then y, then x split of gsq (each pair
+                    // sums to its parent)
                     const float g1 = gsq * fz, g0 = gsq - g1;
                     const float g01 = g0 * fy, g00 = g0 - g01, g11 = g1 * fy, g10 = g1 - g11;
                     const float a00 = g00 * fx, a01 = g01 * fx, a10 = g10 * fx, a11 = g11 * fx;
@@ -382,17 +382,37 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             const bool staged = !empty && B0 * B1 * B2 <= P.cap;
             if (!staged && empty) continue;   // no sample of the tile can lie in this slab
             const int nbox = staged ? B0 * B1 * B2 : 0;
-            const unsigned m0 = 0xffffffffu / (unsigned)B0 + 1u, m1 = 0xffffffffu / (unsigned)B1 + 1u;
+            // box element i -> volume index, with the edge clamp.  i / B is (i + 0.5) * (1/B)
+            // rounded down in FP32: the quotient is below 2^13 and the nearest integer is at
+            // least 0.5/B away, far more than the rounding error of the product.
+            const float rB0 = 1.0f / (float)B0, rB1 = 1.0f / (float)B1;
+            auto box_voxel = [&](int i) {
+                const int r = (int)fmaf((float)i, rB0, 0.5f * rB0);
+                const int x = min(max(lo[0] + (i - __mul24(r, B0)), 0), b.W - 1);
+                const int z = (int)fmaf((float)r, rB1, 0.5f * rB1);
+                const int y = min(max(lo[1] + (r - __mul24(z, B1)), 0), b.H - 1);
+                const int zz = min(max(lo[2] + z, 0), b.D - 1);
+                return (int)(__umul24((unsigned)zz, (unsigned)b.H) + (unsigned)y) * b.W + x;
+            };
             if (staged) {
                 __syncthreads();
-                for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {
-                    const int r = box_index_div((unsigned)i, (unsigned)B0, m0);
-                    const int x = min(max(lo[0] + (i - r * B0), 0), b.W - 1);
-                    const int z = box_index_div((unsigned)r, (unsigned)B1, m1);
-                    const int y = min(max(lo[1] + (r - z * B1), 0), b.H - 1);
-                    const int zz = min(max(lo[2] + z, 0), b.D - 1);
-                    vbox[i] = P.vol[(zz * b.H + y) * b.W + x];
-                    if (MODE == 2) gboxu[i] = 0u;
+                // four loads in flight per lane before their LDS stores
+                constexpr int NB = 4;
+                for (int i0 = threadIdx.x; i0 < nbox; i0 += NB * SL_THREADS) {
+                    float vv[NB];
+#pragma unroll
+                    for (int k = 0; k < NB; ++k) {
+                        const int i = i0 + k * SL_THREADS;
+                        vv[k] = i < nbox ? P.vol[box_voxel(i)] : 0.f;
+                    }
+#pragma unroll
+                    for (int k = 0; k < NB; ++k) {
+                        const int i = i0 + k * SL_THREADS;
+                        if (i < nbox) {
+                            vbox[i] = vv[k];
+                            if (MODE == 2) gboxu[i] = 0u;
+                        }
+                    }
                 }
                 __syncthreads();
             }
@@ -411,18 +431,14 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                     const int ival = (int)gboxu[i];
                     if (ival != 0) {
                         const float gval = (float)ival * inv_scale;
-                        const int r = box_index_div((unsigned)i, (unsigned)B0, m0);
-                        const int x = min(max(lo[0] + (i - r * B0), 0), b.W - 1);
-                        const int z = box_index_div((unsigned)r, (unsigned)B1, m1);
-                        const int y = min(max(lo[1] + (r - z * B1), 0), b.H - 1);
-                        const int zz = min(max(lo[2] + z, 0), b.D - 1);
+                        const int vi = box_voxel(i);
                         if (P.gacc) {
                             const long long q = (long long)ival;
                             const int sh = E - e_tile;
                             const long long iv = sh >= 0 ? (q << min(sh, 62)) : (q >> min(-sh, 62));
-                            if (iv != 0) atomicAdd(P.gacc + (zz * b.H + y) * b.W + x, (unsigned long long)iv);
+                            if (iv != 0) atomicAdd(P.gacc + vi, (unsigned long long)iv);
                         } else {
-                            atomicAdd(P.gvol + (zz * b.H + y) * b.W + x, gval);
+                            atomicAdd(P.gvol + vi, gval);
                         }
                     }
                 }
@@ -439,6 +455,13 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     if (MODE == 0) {
         if (inr) P.proj[((int64_t)a * P.U + u) * P.V + v] = hit ? integral * P.step : 0.f;
         return;
+    }
+    {
+        // the constant factors of the geometry sums: upstream weight * step (1 in the calibration
+        // modes, whose weight comes below) and 1/voxel size; sum over samples of grad . d
+        const float sc = (MODE >= 3 ? 1.0f : gs) * P.inv_vs;
+        gtmin = sc * fmaf(ray.dz, goz, fmaf(ray.dy, goy, ray.dx * gox));
+        gox *= sc; goy *= sc; goz *= sc; gdx *= sc; gdy *= sc; gdz *= sc;
     }
     float loss = 0.f, resid = 0.f;
     if (MODE == 4 && inr) {
@@ -527,13 +550,13 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     }
 }
 
-template <int MODE, bool DIV, bool ZOFF>
+template <int MODE, bool DIV, bool ZOFF, int CAPC = 0>
 __global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
     // Workgroups are dispatched round-robin over the 8 XCDs; permute so that consecutive tiles of a
     // view share an XCD (and its L2).
     const int xcd = blockIdx.x & 7;
     const int tile = ((P.nblocks & 7) == 0) ? xcd * (P.nblocks >> 3) + (blockIdx.x >> 3) : (int)blockIdx.x;
-    slab_tile<MODE, DIV, ZOFF>(P, tile);
+    slab_tile<MODE, DIV, ZOFF, CAPC>(P, tile);
 }
 
 // Persistent calibration kernel: workgroups pull tiles of all problems from one counter.
@@ -587,6 +610,12 @@ int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream)
     } else if (mode == 1) {
         if (zoff) fp_slab_kernel<1, false, true><<<grid, block, shmem, stream>>>(args);
         else fp_slab_kernel<1, false, false><<<grid, block, shmem, stream>>>(args);
+    } else if (zoff && args.cap == 4096) {
+        fp_slab_kernel<2, false, true, 4096><<<grid, block, shmem, stream>>>(args);
+    } else if (zoff && args.cap == 2048) {
+        fp_slab_kernel<2, false, true, 2048><<<grid, block, shmem, stream>>>(args);
+    } else if (zoff && args.cap == 8192) {
+        fp_slab_kernel<2, false, true, 8192><<<grid, block, shmem, stream>>>(args);
     } else {
         if (zoff) fp_slab_kernel<2, false, true><<<grid, block, shmem, stream>>>(args);
         else fp_slab_kernel<2, false, false><<<grid, block, shmem, stream>>>(args);

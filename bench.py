@@ -627,7 +627,9 @@ def main():
                    "pixel_mm": w["pixel"], "sod_mm": w["sod"], "sid_mm": w["sid"], "voxel_mm": w["voxel"],
                    "step_mm": w["step"], "geometry_perturbed": w["perturb"], "samples": S,
                    "t_fwd_ms": t_fwd * 1e3, ("t_fdk_ms" if fdk else "t_adj_ms"): t_adj * 1e3,
-                   "fp_tuning": list(_native.get_fp_tuning()), "deterministic": args.deterministic},
+                   "fp_tuning": {"gather": list(_native.get_fp_tuning(_native.TUNING_GATHER)),
+                                 "adjoint": list(_native.get_fp_tuning(_native.TUNING_SCATTER))},
+                   "deterministic": args.deterministic},
         "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": None if (args.fp_tuning or args.deterministic)

@@ -43,6 +43,8 @@ _SIGNATURES = {
     "cbct_fp_set_deterministic": ([_INT], _INT),
     "cbct_fp_get_deterministic": ([], _INT),
     "cbct_fp_get_tuning": ([_P], _INT),
+    "cbct_fp_set_tuning_op": ([_INT, _INT, _INT, _INT, _INT], _INT),
+    "cbct_fp_get_tuning_op": ([_INT, _P], _INT),
     "cbct_bp_workspace_bytes": ([_I64, _I64, _I64, _I64], _SZ),
     "cbct_bp_forward": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
                         _P, _P, _SZ, _P], _INT),
@@ -125,14 +127,24 @@ def _require_gpu(t, what):
         raise RuntimeError(f"{what} must be a GPU tensor (ROCm devices appear as 'cuda')")
 
 
-def set_fp_tuning(use_slab, slab_thickness, lds_floats, layout):
-    """Select the projector kernels for subsequent calls (same results for every setting)."""
-    _check(lib().cbct_fp_set_tuning(int(use_slab), int(slab_thickness), int(lds_floats), int(layout)))
+# Tuning slots: 0 = kernels that only gather (forward, geometry-only adjoint, calibration march),
+# 1 = the adjoint with the volume gradient.
+TUNING_GATHER, TUNING_SCATTER = 0, 1
 
 
-def get_fp_tuning():
+def set_fp_tuning(use_slab, slab_thickness, lds_floats, layout, op=None):
+    """Select the projector kernels for subsequent calls (same results for every setting), for
+    one tuning slot (``op`` 0 or 1) or both (``op=None``)."""
+    args = (int(use_slab), int(slab_thickness), int(lds_floats), int(layout))
+    if op is None:
+        _check(lib().cbct_fp_set_tuning(*args))
+    else:
+        _check(lib().cbct_fp_set_tuning_op(int(op), *args))
+
+
+def get_fp_tuning(op=TUNING_GATHER):
     out = (ctypes.c_int * 4)()
-    _check(lib().cbct_fp_get_tuning(ctypes.cast(out, ctypes.c_void_p)))
+    _check(lib().cbct_fp_get_tuning_op(int(op), ctypes.cast(out, ctypes.c_void_p)))
     return tuple(out)
 
 
@@ -162,18 +174,21 @@ class deterministic:
 
 
 class fp_tuning:
-    """Context manager that applies a projector tuning and restores the previous one."""
+    """Context manager that applies a projector tuning (to one slot, or both with ``op=None``)
+    and restores both slots afterwards."""
 
-    def __init__(self, use_slab, slab_thickness=8, lds_floats=4096, layout=1):
+    def __init__(self, use_slab, slab_thickness=8, lds_floats=4096, layout=1, op=None):
         self.new = (use_slab, slab_thickness, lds_floats, layout)
+        self.op = op
 
     def __enter__(self):
-        self.old = get_fp_tuning()
-        set_fp_tuning(*self.new)
+        self.old = (get_fp_tuning(TUNING_GATHER), get_fp_tuning(TUNING_SCATTER))
+        set_fp_tuning(*self.new, op=self.op)
         return self
 
     def __exit__(self, *exc):
-        set_fp_tuning(*self.old)
+        set_fp_tuning(*self.old[0], op=TUNING_GATHER)
+        set_fp_tuning(*self.old[1], op=TUNING_SCATTER)
         return False
 
 

@@ -27,7 +27,15 @@ from . import _native, calibration
 SLAB_THICKNESS = (4, 8, 12, 16)
 LDS_FLOATS = (2048, 4096, 8192)
 LAYOUT = (0, 1)
-DEFAULT = (8, 4096, 1)
+# The library's built-in settings per tuning slot (gathering kernels, volume-gradient adjoint).
+DEFAULT_GATHER = (16, 4096, 0)
+DEFAULT = DEFAULT_SCATTER = (12, 4096, 1)
+_SLOT = {"forward": 0, "geometry": 0, "calibration": 0, "adjoint": 1}
+
+
+def default_for(op):
+    """The built-in setting of the slot that ``op`` runs in."""
+    return DEFAULT_SCATTER if _SLOT[op] == 1 else DEFAULT_GATHER
 
 
 def candidates():
@@ -124,7 +132,7 @@ def tune_projector(volume, geometry, pixel_size, U, V, step_size, voxel_size, op
     best = min(timings, key=timings.get)
     if use_cache and space is None:
         _store(key, {"best": list(best), "ms_per_view": timings[best] / n,
-                     "default_ms_per_view": timings.get(DEFAULT, float("nan")) / n})
+                     "default_ms_per_view": timings.get(default_for(op), float("nan")) / n})
     return best
 
 
@@ -150,11 +158,12 @@ def tune_calibration(problem, views=8, reps=2, use_cache=True, space=None):
     best = min(timings, key=timings.get)
     if use_cache and space is None:
         _store(key, {"best": list(best), "ms_per_view": timings[best] / len(idx),
-                     "default_ms_per_view": timings.get(DEFAULT, float("nan")) / len(idx)})
+                     "default_ms_per_view": timings.get(DEFAULT_GATHER, float("nan")) / len(idx)})
     problem.tuning = best
     return best
 
 
-def apply(tuning):
-    """Make ``tuning`` the process-wide projector setting."""
-    _native.set_fp_tuning(1, *tuning)
+def apply(tuning, op=None):
+    """Make ``tuning`` the process-wide projector setting of the slot that ``op`` ("forward",
+    "geometry", "adjoint", "calibration") runs in, or of both slots (``op=None``)."""
+    _native.set_fp_tuning(1, *tuning, op=None if op is None else _SLOT[op])

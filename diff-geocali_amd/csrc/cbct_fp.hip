@@ -207,14 +207,19 @@ __global__ __launch_bounds__(FP_THREADS) void fp_count_kernel(
     if (threadIdx.x == 0) atomicAdd(count, (red[0] + red[1]) + (red[2] + red[3]));
 }
 
-// Process-wide tuning of the projector pair (set by cbct_fp_set_tuning).
+// Process-wide tuning of the projector kernels (cbct_fp_set_tuning / cbct_fp_set_tuning_op), one
+// setting for the kernels that only gather (forward, geometry-only adjoint, calibration march) and
+// one for the adjoint that also scatters the volume gradient into LDS: gathers favour 8x8 lane
+// blocks (neighbouring taps share banks and cache lines), the LDS integer atomics favour the
+// interleaved layout (fewer lanes of a wave on one bank).  Defaults measured on the 256^3 x 720
+// views, 768x640 workload.
 struct FpTuning {
-    int use_slab = 1;     // slab-staged kernels (1) or direct kernels (0)
-    int T = 8;            // slab thickness in voxel planes
-    int cap = 4096;       // LDS floats per staged box
-    int layout = 1;       // lane layout of a tile
+    int use_slab;         // slab-staged kernels (1) or direct kernels (0)
+    int T;                // slab thickness in voxel planes
+    int cap;              // LDS floats per staged box
+    int layout;           // lane layout of a tile
 };
-static FpTuning g_tuning;
+static FpTuning g_tuning_op[2] = {{1, 16, 4096, 0}, {1, 12, 4096, 1}};
 static int g_deterministic = 0;   // volume gradient through order-independent int64 accumulation
 
 // ---- deterministic volume gradient helpers
@@ -305,10 +310,11 @@ extern "C" int cbct_fp_forward(const float* volume, int64_t D, int64_t H, int64_
     const int tu = (int)ceil_div(U, FP_TILE), tv = (int)ceil_div(V, FP_TILE);
     const int nblocks = (int)(A * tu * tv);
     const float inv_vs = 1.0f / voxel_size;
-    if (g_tuning.use_slab) {
+    const FpTuning& tf = g_tuning_op[0];
+    if (tf.use_slab) {
         const SlabArgs args{volume, nullptr, frames, projections, nullptr, nullptr, b, (int)U, (int)V, tu, tv,
-                            nblocks, pixel_size, step_size, voxel_size, inv_vs, g_tuning.T, g_tuning.cap,
-                            g_tuning.layout};
+                            nblocks, pixel_size, step_size, voxel_size, inv_vs, tf.T, tf.cap,
+                            tf.layout};
         return launch_fp_slab(0, !is_pow2(voxel_size), args, stream);
     }
     if (is_pow2(voxel_size))
@@ -360,7 +366,8 @@ extern "C" int cbct_fp_backward(const float* grad_projection, const float* volum
     CBCT_CHECK_ARG(!compute_grad_volume || grad_volume != nullptr,
                    "grad_volume must be non-null when compute_grad_volume is set");
     const bool det = g_deterministic && compute_grad_volume;
-    CBCT_CHECK_ARG(!det || g_tuning.use_slab, "the deterministic volume gradient needs the slab kernels");
+    const FpTuning& tb = g_tuning_op[compute_grad_volume ? 1 : 0];
+    CBCT_CHECK_ARG(!det || tb.use_slab, "the deterministic volume gradient needs the slab kernels");
     const size_t base_bytes = cbct_fp_workspace_bytes(A, U, V);
     CBCT_CHECK_ARG(workspace != nullptr &&
                    workspace_bytes >= base_bytes + (det ? det_extra_bytes(D, H, W) : 0),
@@ -393,10 +400,10 @@ extern "C" int cbct_fp_backward(const float* grad_projection, const float* volum
     const int tu = (int)ceil_div(U, FP_TILE), tv = (int)ceil_div(V, FP_TILE);
     const int nblocks = (int)(A * tu * tv);
     const float inv_vs = 1.0f / voxel_size;
-    if (g_tuning.use_slab) {
+    if (tb.use_slab) {
         SlabArgs args{volume, grad_projection, frames, nullptr, compute_grad_volume ? grad_volume : nullptr,
                       partials, b, (int)U, (int)V, tu, tv, nblocks, pixel_size, step_size, voxel_size,
-                      inv_vs, g_tuning.T, g_tuning.cap, g_tuning.layout};
+                      inv_vs, tb.T, tb.cap, tb.layout};
         args.gacc = gacc;
         args.gexp = gexp;
         if ((st = launch_fp_slab(compute_grad_volume ? 2 : 1, false, args, stream))) return st;
@@ -448,23 +455,31 @@ extern "C" int cbct_fp_count_samples(int64_t D, int64_t H, int64_t W,
     return CBCT_OK;
 }
 
-extern "C" int cbct_fp_set_tuning(int use_slab, int slab_thickness, int lds_floats, int layout) {
+extern "C" int cbct_fp_set_tuning_op(int op, int use_slab, int slab_thickness, int lds_floats, int layout) {
+    CBCT_CHECK_ARG(op == 0 || op == 1, "op must be 0 (gathering kernels) or 1 (volume-gradient adjoint)");
     CBCT_CHECK_ARG(use_slab == 0 || use_slab == 1, "use_slab must be 0 or 1");
     CBCT_CHECK_ARG(slab_thickness >= 1 && slab_thickness <= 64, "slab_thickness must be in [1, 64]");
     CBCT_CHECK_ARG(lds_floats >= 256 && lds_floats <= 8192, "lds_floats must be in [256, 8192]");
     CBCT_CHECK_ARG(layout == 0 || layout == 1, "layout must be 0 or 1");
-    g_tuning.use_slab = use_slab;
-    g_tuning.T = slab_thickness;
-    g_tuning.cap = lds_floats;
-    g_tuning.layout = layout;
+    g_tuning_op[op] = FpTuning{use_slab, slab_thickness, lds_floats, layout};
     return CBCT_OK;
 }
 
-extern "C" int cbct_fp_get_tuning(int* out4) {
+extern "C" int cbct_fp_get_tuning_op(int op, int* out4) {
+    CBCT_CHECK_ARG(op == 0 || op == 1, "op must be 0 (gathering kernels) or 1 (volume-gradient adjoint)");
     CBCT_CHECK_ARG(out4 != nullptr, "out4 must be non-null");
-    out4[0] = g_tuning.use_slab;
-    out4[1] = g_tuning.T;
-    out4[2] = g_tuning.cap;
-    out4[3] = g_tuning.layout;
+    const FpTuning& t = g_tuning_op[op];
+    out4[0] = t.use_slab;
+    out4[1] = t.T;
+    out4[2] = t.cap;
+    out4[3] = t.layout;
     return CBCT_OK;
 }
+
+extern "C" int cbct_fp_set_tuning(int use_slab, int slab_thickness, int lds_floats, int layout) {
+    int st = cbct_fp_set_tuning_op(0, use_slab, slab_thickness, lds_floats, layout);
+    if (st == CBCT_OK) st = cbct_fp_set_tuning_op(1, use_slab, slab_thickness, lds_floats, layout);
+    return st;
+}
+
+extern "C" int cbct_fp_get_tuning(int* out4) { return cbct_fp_get_tuning_op(0, out4); }

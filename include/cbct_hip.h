@@ -102,12 +102,18 @@ int cbct_fp_count_samples(int64_t D, int64_t H, int64_t W,
  * (1) or the direct kernels (0); slab_thickness is the number of voxel planes
  * per staged slab, lds_floats the LDS capacity of one staged box, layout the
  * lane-to-pixel layout of a 16x16 ray tile (0: 8x8 blocks per wave, 1: waves
- * interleaved with stride 2).  Every setting gives the same projections bit
- * for bit; geometry gradients are bit-identical for a given layout (another
- * layout only changes their FP32 summation order).  Not synchronised with
- * calls in flight on other host threads. */
+ * interleaved with stride 2).  There is one setting per operation class:
+ * op 0 for the kernels that only gather (cbct_fp_forward, cbct_fp_backward
+ * without the volume gradient, and the default of the calibration march), op 1
+ * for cbct_fp_backward with the volume gradient; cbct_fp_set_tuning sets both
+ * and cbct_fp_get_tuning reads op 0.  Every setting gives the same projections
+ * bit for bit; geometry gradients are bit-identical for a given layout
+ * (another layout only changes their FP32 summation order).  Not synchronised
+ * with calls in flight on other host threads. */
 int cbct_fp_set_tuning(int use_slab, int slab_thickness, int lds_floats, int layout);
 int cbct_fp_get_tuning(int* out4);
+int cbct_fp_set_tuning_op(int op, int use_slab, int slab_thickness, int lds_floats, int layout);
+int cbct_fp_get_tuning_op(int op, int* out4);
 
 /* ---- weighted backprojector: src_cbct_backprojector_cuda --------------- */
 

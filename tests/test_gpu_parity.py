@@ -188,7 +188,9 @@ def test_c1_all_nine_gradients_and_native_modules():
         e64 = cases.rel_l2(cases.to_np(grads[k + 1]), ggeo[k])
         print(f"c1 geometry gradient {k}: rel L2 vs FP32 schedule {e32:.3e}, vs FP64 {e64:.3e}")
         assert e32 <= 1e-5 and e64 <= 2e-2, k
-        np.testing.assert_array_equal(cases.to_np(g_nov[k + 1]), cases.to_np(grads[k + 1]))
+        # the geometry-only adjoint runs in the gathering kernels' tuning slot (another lane
+        # layout), which changes only the FP32 summation order of the per-tile sums
+        assert cases.rel_l2(cases.to_np(g_nov[k + 1]), cases.to_np(grads[k + 1])) <= 1e-6
     vb = bpn.forward(proj, *tg, c["pixel"], *c["shape"], c["voxel"])
     gv = rng.standard_normal(vb.shape).astype(np.float32)
     bg = bpn.backward(torch.tensor(gv, device=DEV), proj, *tg, c["pixel"], c["voxel"])

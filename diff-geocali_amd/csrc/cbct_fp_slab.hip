@@ -210,18 +210,21 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         use_taps(tp, frac(x), frac(y), frac(z), true);
     };
 
+    // (axis k of a 3-vector by selects: a runtime array index would put the array in scratch)
+    auto pick = [](const float* v3, int k) { return k == 0 ? v3[0] : (k == 1 ? v3[1] : v3[2]); };
+    auto pick_i = [](const int* v3, int k) { return k == 0 ? v3[0] : (k == 1 ? v3[1] : v3[2]); };
     // ---- principal axis of the tile (uniform over the workgroup)
     const int u_lo = tu * SL_TILE, v_lo = tv * SL_TILE;
     const int u_hi = min(u_lo + SL_TILE - 1, P.U - 1), v_hi = min(v_lo + SL_TILE - 1, P.V - 1);
     float cr[3];
     pixel_dir(f, 0.5f * (float)(u_lo + u_hi), 0.5f * (float)(v_lo + v_hi), P.pix, cr);
     int p = 0;
-    if (fabsf(cr[1]) > fabsf(cr[p])) p = 1;
-    if (fabsf(cr[2]) > fabsf(cr[p])) p = 2;
+    if (fabsf(cr[1]) > fabsf(cr[0])) p = 1;
+    if (fabsf(cr[2]) > fabsf(pick(cr, p))) p = 2;
     // every lane computed the same axis and direction; make that explicit (scalar registers,
     // uniform branches in the march loop)
     p = __builtin_amdgcn_readfirstlane(p);
-    const int fwd = __builtin_amdgcn_readfirstlane(cr[p] > 0.f ? 1 : 0);
+    const int fwd = __builtin_amdgcn_readfirstlane(pick(cr, p) > 0.f ? 1 : 0);
     const float dirf = fwd ? 1.f : -1.f;
     float corner[4][3];
     pixel_dir(f, (float)u_lo, (float)v_lo, P.pix, corner[0]);
@@ -231,7 +234,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     bool slab_ok = true;
     for (int j = 0; j < 4; ++j) {
         const float n2 = corner[j][0] * corner[j][0] + corner[j][1] * corner[j][1] + corner[j][2] * corner[j][2];
-        const float cp = corner[j][p] * dirf;
+        const float cp = pick(corner[j], p) * dirf;
         slab_ok = slab_ok && cp > 0.f && cp * cp >= 0.0625f * n2;
     }
     const int Nax[3] = {b.W, b.H, b.D};
@@ -264,25 +267,33 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     // planes lie in front of the source exactly when (q - xs_p) has the direction's sign.
     float sl1[4], sl2[4];
     for (int j = 0; j < 4; ++j) {
-        sl1[j] = corner[j][o1] / corner[j][p];
-        sl2[j] = corner[j][o2] / corner[j][p];
+        const float cpj = pick(corner[j], p);
+        sl1[j] = pick(corner[j], o1) / cpj;
+        sl2[j] = pick(corner[j], o2) / cpj;
     }
+    const float xsp = pick(xs, p), xs1 = pick(xs, o1), xs2 = pick(xs, o2);
+    const int Nx1 = pick_i(Nax, o1), Nx2 = pick_i(Nax, o2);
+    const int Np = pick_i(Nax, p);
     auto slab_box = [&](int s_lo, int s_hi, int lo[3], int hi[3]) {
-        const float q0 = (float)s_lo - xs[p], q1 = (float)(s_hi + 1) - xs[p];
+        const float q0 = (float)s_lo - xsp, q1 = (float)(s_hi + 1) - xsp;
         const bool front = q0 * dirf > 0.f && q1 * dirf > 0.f;
         float mn1 = FLT_MAX, mx1 = -FLT_MAX, mn2 = FLT_MAX, mx2 = -FLT_MAX;
         for (int j = 0; j < 4; ++j) {
-            const float a1 = fmaf(q0, sl1[j], xs[o1]), b1 = fmaf(q1, sl1[j], xs[o1]);
-            const float a2 = fmaf(q0, sl2[j], xs[o2]), b2 = fmaf(q1, sl2[j], xs[o2]);
+            const float a1 = fmaf(q0, sl1[j], xs1), b1 = fmaf(q1, sl1[j], xs1);
+            const float a2 = fmaf(q0, sl2[j], xs2), b2 = fmaf(q1, sl2[j], xs2);
             mn1 = fminf(mn1, fminf(a1, b1)); mx1 = fmaxf(mx1, fmaxf(a1, b1));
             mn2 = fminf(mn2, fminf(a2, b2)); mx2 = fmaxf(mx2, fmaxf(a2, b2));
         }
-        lo[p] = max(-1, s_lo);
-        hi[p] = min(Nax[p], s_hi + 1);
-        lo[o1] = (int)fminf(fmaxf(floorf(mn1) - 1.f, -1.f), (float)Nax[o1]);
-        hi[o1] = (int)fmaxf(fminf(floorf(mx1) + 2.f, (float)Nax[o1]), -1.f);
-        lo[o2] = (int)fminf(fmaxf(floorf(mn2) - 1.f, -1.f), (float)Nax[o2]);
-        hi[o2] = (int)fmaxf(fminf(floorf(mx2) + 2.f, (float)Nax[o2]), -1.f);
+        const int lp = max(-1, s_lo), hp = min(Np, s_hi + 1);
+        const int l1 = (int)fminf(fmaxf(floorf(mn1) - 1.f, -1.f), (float)Nx1);
+        const int h1 = (int)fmaxf(fminf(floorf(mx1) + 2.f, (float)Nx1), -1.f);
+        const int l2 = (int)fminf(fmaxf(floorf(mn2) - 1.f, -1.f), (float)Nx2);
+        const int h2 = (int)fmaxf(fminf(floorf(mx2) + 2.f, (float)Nx2), -1.f);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = k == p ? lp : (k == o1 ? l1 : l2);
+            hi[k] = k == p ? hp : (k == o1 ? h1 : h2);
+        }
         return front && lo[0] < hi[0] && lo[1] < hi[1] && lo[2] < hi[2];
     };
 
@@ -352,7 +363,6 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     };
 
     if (slab_ok) {
-        const int Np = Nax[p];
         // Slabs cover the principal floor coordinates -1 .. Np-1 in the rays' direction; a slab is
         // halved until its box fits the LDS capacity (uniform decision over the workgroup).
         int next = dirf > 0.f ? -1 : Np - 1;
@@ -551,7 +561,8 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
 }
 
 template <int MODE, bool DIV, bool ZOFF, int CAPC = 0>
-__global__ __launch_bounds__(SL_THREADS) void fp_slab_kernel(SlabArgs P) {
+__global__ __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 5 : 1)))
+void fp_slab_kernel(SlabArgs P) {
     // Workgroups are dispatched round-robin over the 8 XCDs; permute so that consecutive tiles of a
     // view share an XCD (and its L2).
     const int xcd = blockIdx.x & 7;

@@ -52,8 +52,8 @@ DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 # Physical HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs of
 # `bench.py --pmc-probe` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB copy in the
 # same runs): adjoint fp_slab_kernel<2> at 256^3 x 720 views, 768x640 detector.
-MEASURED_TRAFFIC = {("geometry-gradients-256c-720v", "adjoint"): 95_506_658 * 1024 * 2.0 + 134_343_800 * 1024,
-                    ("geometry-gradients-256c-720v", "forward"): 111_956_382 * 1024 * 2.0 + 1_501_411 * 1024}
+MEASURED_TRAFFIC = {("geometry-gradients-256c-720v", "adjoint"): 98_182_639 * 1024 * 2.0 + 130_501_291 * 1024,
+                    ("geometry-gradients-256c-720v", "forward"): 112_954_100 * 1024 * 2.0 + 1_527_437 * 1024}
 PHANTOM_SEED = 20261203
 PERTURB_SEED = 7
 
@@ -120,13 +120,27 @@ def copy_rate_gbps(torch, device):
     return 2 * n * 4 / t / 1e9
 
 
+def sample_views(A, n=256):
+    """Up to n distinct views for a CPU-baseline sample, in an order whose every prefix is spread
+    over the scan (bit-reversed fractions of the orbit), so stopping at the time budget still
+    covers the whole orbit."""
+    bits = max(1, (max(A, 2) - 1).bit_length())
+    out, seen = [], set()
+    for i in range(1 << bits):
+        a = int(int(format(i, f"0{bits}b")[::-1], 2) * A >> bits)
+        if a not in seen:
+            seen.add(a)
+            out.append(a)
+    return out[:min(A, n)]
+
+
 def cpu_baseline(w, vol_np, geom_np, budget_s):
     """FP32 CPU oracle forward + adjoint on uniformly spaced views until the budget is used."""
     import numpy as np
 
     import oracle
     A = w["A"]
-    order = [int(round(i * A / 8)) % A for i in range(8)]
+    order = sample_views(A)
     threads = oracle.max_threads()
     rng = np.random.default_rng(11)
     done, t_total = 0, 0.0
@@ -155,7 +169,7 @@ def cpu_baseline_fdk(w, vol_np, geom_np, budget_s):
 
     import oracle
     A, D = w["A"], w["D"]
-    order = [int(round(i * A / 8)) % A for i in range(8)]
+    order = sample_views(A)
     threads = oracle.max_threads()
     done, t_total = 0, 0.0
     for a in order:
@@ -180,7 +194,7 @@ def cpu_baseline_bp(w, geom_np, proj, gvol_np, budget_s):
 
     import oracle
     A, D = w["A"], w["D"]
-    order = [int(round(i * A / 8)) % A for i in range(8)]
+    order = sample_views(A)
     threads = oracle.max_threads()
     done, t_total = 0, 0.0
     for a in order:

@@ -263,6 +263,11 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
             const float dt_sy = f.c * qml * iq2;
             const float dU_sy = fmaf(du, dt_sy, (1.0f - t) * f.s);
             const float dW_sy = dW_m * f.c;
+            // Every geometry term of a voxel is a combination, with coefficients constant along the
+            // column, of four per-voxel products: gb*Iv*dv, gb*Iu, gb*Iv and g*interp.  The column
+            // sums those four and combines them once (the same terms as :214-292, summed in
+            // another order).
+            float P1 = 0.f, P2 = 0.f, P3 = 0.f, P4 = 0.f;
             const int row = (iu - u_lo) * nv;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -291,21 +296,26 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
                         atomicAdd(gproj + k00 + 1, gb * mu * fv);
                         atomicAdd(gproj + k00 + g.V + 1, gb * fu * fv);
                     }
-                    // sid (:214-219)
-                    s1 += gb * fmaf(Iv, dv * iq * ip, Iu * du * iq * ip);
-                    // sod (:222-230)
-                    s2 += gv[j][k] * fmaf(gw, fmaf(Iv, dv * dt_sod * ip, Iu * du * dt_sod * ip), dW_m * interp);
-                    // beta (:233-250)
-                    s0 += gv[j][k] * fmaf(gw, fmaf(Iv, dv * dt_b * ip, Iu * dU_b * ip), dW_b * interp);
-                    // detector centres (:252-253)
-                    s3 += gb * Iu;
-                    s4 += gb * Iv;
-                    // source offsets (:255-292)
-                    s5 += gv[j][k] * fmaf(gw, fmaf(Iv, dv * dt_sx * ip, Iu * dU_sx * ip), dW_sx * interp);
-                    s6 += gv[j][k] * fmaf(gw, fmaf(Iv, dv * dt_sy * ip, Iu * dU_sy * ip), dW_sy * interp);
-                    s7 += gb * Iv * (1.0f - t) * ip;
+                    const float gbIv = gb * Iv;
+                    P1 = fmaf(gbIv, dv, P1);
+                    P2 = fmaf(gb, Iu, P2);
+                    P3 += gbIv;
+                    P4 = fmaf(gv[j][k], interp, P4);
                 }
             }
+            // sid (:214-219): gb (Iv dv + Iu du) / q / pix
+            s1 += fmaf(P1, iq * ip, P2 * (du * iq * ip));
+            // sod (:222-230)
+            s2 += fmaf(P1, dt_sod * ip, fmaf(P2, du * dt_sod * ip, P4 * dW_m));
+            // beta (:233-250)
+            s0 += fmaf(P1, dt_b * ip, fmaf(P2, dU_b * ip, P4 * dW_b));
+            // detector centres (:252-253)
+            s3 += P2;
+            s4 += P3;
+            // source offsets (:255-292)
+            s5 += fmaf(P1, dt_sx * ip, fmaf(P2, dU_sx * ip, P4 * dW_sx));
+            s6 += fmaf(P1, dt_sy * ip, fmaf(P2, dU_sy * ip, P4 * dW_sy));
+            s7 += P3 * ((1.0f - t) * ip);
         }
         s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
         s4 = wave_sum(s4); s5 = wave_sum(s5); s6 = wave_sum(s6); s7 = wave_sum(s7);

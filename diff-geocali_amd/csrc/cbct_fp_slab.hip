@@ -84,7 +84,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     // total is at most max|gs| * n_rays * n_smp, where n_smp bounds the samples of one ray inside
     // a voxel's 2x2x2 support (floor(2 sqrt3 vs / step) + 1) and n_rays bounds the tile's rays
     // through that support (pixels within its projected radius, or all 256).
-    float gsq = 0.f, inv_scale = 1.f;
+    float gsq = 0.f, hq = 0.f, inv_scale = 1.f;
     int e_tile = 0;                                   // the tile's fixed-point exponent
     const int E = (MODE == 2 && P.gacc) ? *P.gexp : 0;  // the global exponent (deterministic mode)
     const double scaleE = ldexp(1.0, E);
@@ -128,6 +128,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             e = max(-100, min(100, k - 1));
         }
         gsq = gs * ldexpf(1.f, e);
+        hq = copysignf(0.5f, gsq);
         inv_scale = ldexpf(1.f, -e);
         e_tile = e;
     }
@@ -337,20 +338,21 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 const Taps tp = load_taps_strided(vbox, idx, 1, B0, PL);
                 use_taps(tp, fx, fy, fz, gl);
                 if (MODE == 2 && gl) {
-                    // the scaled trilinear weights as a z, then y, then x split of gsq (each pair
-                    // sums to its parent)
+                    // the scaled trilinear weights as a z, then y split of gsq, then x weights
+                    // plus half a unit of gsq's sign (every weight has it), so that truncation
+                    // (one conversion) rounds to nearest, halves away from zero
                     const float g1 = gsq * fz, g0 = gsq - g1;
                     const float g01 = g0 * fy, g00 = g0 - g01, g11 = g1 * fy, g10 = g1 - g11;
-                    const float a00 = g00 * fx, a01 = g01 * fx, a10 = g10 * fx, a11 = g11 * fx;
+                    const float mx = 1.0f - fx;
                     unsigned* qi = gboxu + idx;
-                    atomicAdd(qi, (unsigned)__float2int_rn(g00 - a00));
-                    atomicAdd(qi + 1, (unsigned)__float2int_rn(a00));
-                    atomicAdd(qi + B0, (unsigned)__float2int_rn(g01 - a01));
-                    atomicAdd(qi + B0 + 1, (unsigned)__float2int_rn(a01));
-                    atomicAdd(qi + PL, (unsigned)__float2int_rn(g10 - a10));
-                    atomicAdd(qi + PL + 1, (unsigned)__float2int_rn(a10));
-                    atomicAdd(qi + PL + B0, (unsigned)__float2int_rn(g11 - a11));
-                    atomicAdd(qi + PL + B0 + 1, (unsigned)__float2int_rn(a11));
+                    atomicAdd(qi, (unsigned)(int)fmaf(g00, mx, hq));
+                    atomicAdd(qi + 1, (unsigned)(int)fmaf(g00, fx, hq));
+                    atomicAdd(qi + B0, (unsigned)(int)fmaf(g01, mx, hq));
+                    atomicAdd(qi + B0 + 1, (unsigned)(int)fmaf(g01, fx, hq));
+                    atomicAdd(qi + PL, (unsigned)(int)fmaf(g10, mx, hq));
+                    atomicAdd(qi + PL + 1, (unsigned)(int)fmaf(g10, fx, hq));
+                    atomicAdd(qi + PL + B0, (unsigned)(int)fmaf(g11, mx, hq));
+                    atomicAdd(qi + PL + B0 + 1, (unsigned)(int)fmaf(g11, fx, hq));
                 }
             }
             if (go && !gl) {      // a divergent branch that no lane takes costs a skip

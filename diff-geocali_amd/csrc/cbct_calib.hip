@@ -341,7 +341,7 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
         calib_frames_kernel<<<(unsigned)ceil_div(total_views, 128), 128, 0, st_>>>(ddev, nproblems, total_views);
         CBCT_LAUNCHED("calibration frames kernel launch");
         int st;
-        if ((st = launch_fp_slab_batched(ddev, nproblems, total_tiles, max_cap, counter, grid, st_))) return st;
+        if ((st = launch_fp_slab_batched(ddev, nproblems, total_tiles, max_cap, solver, counter, grid, st_))) return st;
         if (solver == 1) {
             calib_reduce_kernel<LM_NE><<<(unsigned)total_views, 256, 0, st_>>>(ddev, nproblems, step);
             CBCT_LAUNCHED("calibration reduce kernel launch");

@@ -51,9 +51,10 @@ constexpr int LM_STATE = 8 + LM_NE + 2; // geometry, normal equations, lambda, l
 // mode 0: forward; 1: adjoint (geometry only); 2: adjoint with the volume gradient.
 int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream);
 
-// Mode 3 (fused forward + squared residual + geometry adjoint) over every tile of every problem,
-// persistent workgroups fed by the work counter (which must be zero at launch).
-int launch_fp_slab_batched(const CalibDev* probs, int nprob, int total_tiles, int max_cap,
+// Mode 3 (fused forward + squared residual + geometry adjoint; solver 0) or mode 4 (normal
+// equations; solver 1) over every tile of every problem, persistent workgroups fed by the work
+// counter (which must be zero at launch).  Every problem of a launch uses the same solver.
+int launch_fp_slab_batched(const CalibDev* probs, int nprob, int total_tiles, int max_cap, int solver,
                            unsigned* counter, int grid, hipStream_t stream);
 
 }  // namespace cbct

@@ -573,7 +573,10 @@ void fp_slab_kernel(SlabArgs P) {
 }
 
 // Persistent calibration kernel: workgroups pull tiles of all problems from one counter.
-__global__ __launch_bounds__(SL_THREADS) void fp_slab_batched_kernel(const CalibDev* __restrict__ probs,
+// MODE 3 (Adam: loss gradient) or 4 (Levenberg-Marquardt: normal equations); one launch serves
+// one solver, so each mode is its own kernel with its own register budget.
+template <int MODE>
+__global__ __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void fp_slab_batched_kernel(const CalibDev* __restrict__ probs,
                                                                      int nprob, int total_tiles,
                                                                      unsigned* __restrict__ counter) {
     __shared__ int sh_tile;
@@ -590,20 +593,18 @@ __global__ __launch_bounds__(SL_THREADS) void fp_slab_batched_kernel(const Calib
         }
         const SlabArgs P = probs[lo].s;
         const int t = tile - probs[lo].tile_offset;
-        if (probs[lo].solver == 1) {
-            if (probs[lo].div) slab_tile<4, true>(P, t);
-            else slab_tile<4, false>(P, t);
-        } else {
-            if (probs[lo].div) slab_tile<3, true>(P, t);
-            else slab_tile<3, false>(P, t);
-        }
+        if (probs[lo].div) slab_tile<MODE, true>(P, t);
+        else slab_tile<MODE, false>(P, t);
     }
 }
 
-int launch_fp_slab_batched(const CalibDev* probs, int nprob, int total_tiles, int max_cap,
+int launch_fp_slab_batched(const CalibDev* probs, int nprob, int total_tiles, int max_cap, int solver,
                            unsigned* counter, int grid, hipStream_t stream) {
-    fp_slab_batched_kernel<<<grid, SL_THREADS, (size_t)max_cap * sizeof(float), stream>>>(
-        probs, nprob, total_tiles, counter);
+    const size_t shmem = (size_t)max_cap * sizeof(float);
+    if (solver == 1)
+        fp_slab_batched_kernel<4><<<grid, SL_THREADS, shmem, stream>>>(probs, nprob, total_tiles, counter);
+    else
+        fp_slab_batched_kernel<3><<<grid, SL_THREADS, shmem, stream>>>(probs, nprob, total_tiles, counter);
     CBCT_LAUNCHED("batched calibration kernel launch");
     return CBCT_OK;
 }

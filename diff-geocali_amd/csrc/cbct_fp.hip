@@ -212,14 +212,17 @@ __global__ __launch_bounds__(FP_THREADS) void fp_count_kernel(
 // one for the adjoint that also scatters the volume gradient into LDS: gathers favour 8x8 lane
 // blocks (neighbouring taps share banks and cache lines), the LDS integer atomics favour the
 // interleaved layout (fewer lanes of a wave on one bank).  Defaults measured on the 256^3 x 720
-// views, 768x640 workload.
+// views, 768x640 workload; their LDS capacities are the largest round sizes that keep the
+// occupancy the registers allow (8 workgroups of one 20 KB box per CU for the gathering kernels;
+// 5 workgroups of two 15 KB boxes plus 160 B of reductions for the adjoint -- two 16 KB boxes
+// would leave room for only 4).
 struct FpTuning {
     int use_slab;         // slab-staged kernels (1) or direct kernels (0)
     int T;                // slab thickness in voxel planes
     int cap;              // LDS floats per staged box
     int layout;           // lane layout of a tile
 };
-static FpTuning g_tuning_op[2] = {{1, 16, 4096, 0}, {1, 12, 4096, 1}};
+static FpTuning g_tuning_op[2] = {{1, 16, 5120, 0}, {1, 12, 3840, 1}};
 static int g_deterministic = 0;   // volume gradient through order-independent int64 accumulation
 
 // ---- deterministic volume gradient helpers

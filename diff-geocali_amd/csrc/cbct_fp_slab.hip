@@ -624,6 +624,8 @@ int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream)
     } else if (mode == 1) {
         if (zoff) fp_slab_kernel<1, false, true><<<grid, block, shmem, stream>>>(args);
         else fp_slab_kernel<1, false, false><<<grid, block, shmem, stream>>>(args);
+    } else if (zoff && args.cap == 3840) {
+        fp_slab_kernel<2, false, true, 3840><<<grid, block, shmem, stream>>>(args);
     } else if (zoff && args.cap == 4096) {
         fp_slab_kernel<2, false, true, 4096><<<grid, block, shmem, stream>>>(args);
     } else if (zoff && args.cap == 2048) {

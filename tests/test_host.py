@@ -120,4 +120,5 @@ def test_autotune_cache_round_trip_and_shape_classes(tmp_path, monkeypatch):
     autotune._store(k3, {"best": [8, 8192, 0], "ms_per_view": 0.2})
     data = autotune._load()
     assert data[k1]["best"] == [12, 4096, 1] and data[k3]["best"] == [8, 8192, 0]
-    assert len(autotune.candidates()) == 24 and autotune.DEFAULT in autotune.candidates()
+    assert len(autotune.candidates()) == 40 and autotune.DEFAULT in autotune.candidates()
+    assert autotune.DEFAULT_GATHER in autotune.candidates()

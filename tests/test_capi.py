@@ -62,6 +62,29 @@ def test_invalid_arguments_return_status_without_touching_the_device():
     assert st == 1
 
 
+def test_tuning_slots_set_get_and_validate():
+    """The two tuning slots are independent host state; cbct_fp_set_tuning sets both and invalid
+    settings are rejected with a status (no device involved)."""
+    from cbct_backprojector import autotune
+    old = (_native.get_fp_tuning(_native.TUNING_GATHER), _native.get_fp_tuning(_native.TUNING_SCATTER))
+    try:
+        assert old == ((1,) + autotune.DEFAULT_GATHER, (1,) + autotune.DEFAULT_SCATTER)
+        _native.set_fp_tuning(1, 4, 2048, 1, op=_native.TUNING_GATHER)
+        assert _native.get_fp_tuning(_native.TUNING_GATHER) == (1, 4, 2048, 1)
+        assert _native.get_fp_tuning(_native.TUNING_SCATTER) == old[1]
+        with _native.fp_tuning(0, 8, 4096, 0):
+            assert _native.get_fp_tuning(0) == _native.get_fp_tuning(1) == (0, 8, 4096, 0)
+        assert _native.get_fp_tuning(_native.TUNING_GATHER) == (1, 4, 2048, 1)
+        L = _native.lib()
+        assert L.cbct_fp_set_tuning_op(2, 1, 8, 4096, 1) == 1 and b"op must be" in L.cbct_last_error()
+        assert L.cbct_fp_set_tuning_op(0, 1, 8, 100, 1) == 1 and b"lds_floats" in L.cbct_last_error()
+        with pytest.raises(RuntimeError):
+            _native.set_fp_tuning(1, 0, 4096, 1, op=1)
+    finally:
+        _native.set_fp_tuning(*old[0], op=_native.TUNING_GATHER)
+        _native.set_fp_tuning(*old[1], op=_native.TUNING_SCATTER)
+
+
 def _cpu_fp_args():
     vol = torch.rand(3, 4, 5)
     g = [torch.zeros(2) for _ in range(8)]

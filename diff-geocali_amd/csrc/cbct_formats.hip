@@ -124,6 +124,97 @@ __global__ __launch_bounds__(256) void preprocess_pair_kernel(const T* __restric
     }
 }
 
+// Transposing variant for frames whose rows hold a multiple of 8 pixels and whose columns a
+// multiple of 4: each lane reads 8 consecutive u (16 bytes of uint16) and writes 4 consecutive v
+// (16 bytes), and a workgroup runs NV views of one 64x64 tile, keeping its pixels' dark value and
+// reciprocal flat-dark gain in registers (the fields are read once per NV views instead of once
+// per view).
+template <typename T> struct Oct;
+template <> struct Oct<uint16_t> {
+    static __device__ __forceinline__ void load(const uint16_t* p, float* x) {
+        const uint4 w = *reinterpret_cast<const uint4*>(p);
+        const unsigned q[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { x[2 * k] = (float)(q[k] & 0xffffu); x[2 * k + 1] = (float)(q[k] >> 16); }
+    }
+};
+template <> struct Oct<float> {
+    static __device__ __forceinline__ void load(const float* p, float* x) {
+        const float4 a = *reinterpret_cast<const float4*>(p), b = *reinterpret_cast<const float4*>(p + 4);
+        x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+    }
+};
+
+template <typename T, int NV>
+__global__ __launch_bounds__(256) void preprocess_views_kernel(const T* __restrict__ raw, const float* __restrict__ dark,
+                                                               const float* __restrict__ flat, int A, int V, int U,
+                                                               float tmin, int take_log, float* __restrict__ out) {
+    __shared__ float tile[PT * PT_PAD];          // [u][v] of the current view
+    const int tiles_u = (U + PT - 1) / PT;
+    const int tu = blockIdx.x % tiles_u, tv = blockIdx.x / tiles_u;
+    const int u0 = tu * PT, v0 = tv * PT;
+    const int a0 = blockIdx.y * NV;
+    const int t = threadIdx.x;
+    const int ru = (t & 7) * 8, rv = t >> 3;     // reads: u0+ru .. +7 of rows v0+rv and v0+rv+32
+    const int wv = (t & 15) * 4, wu = t >> 4;    // writes: v0+wv .. +3 of u-rows u0+wu+16h
+    float dd[2][8], gi[2][8];
+    bool rok[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int v = v0 + rv + 32 * h, u = u0 + ru;
+        rok[h] = v < V && u < U;                 // U % 8 == 0: a chunk is wholly inside or outside
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            dd[h][k] = 0.f;
+            gi[h][k] = 1.f;
+            if (rok[h] && take_log) {
+                const int64_t pix = (int64_t)v * U + u + k;
+                const float d = dark ? dark[pix] : 0.f;
+                const float den = (flat ? flat[pix] : 1.f) - d;
+                // a pixel with flat <= dark: t = 0 -> max(t, t_min) = t_min
+                dd[h][k] = den > 0.f ? d : 0.f;
+                gi[h][k] = den > 0.f ? 1.0f / den : 0.f;
+            }
+        }
+    }
+    // the next view's raw values are loaded while the current view is written out
+    float xn[2][8];
+    auto load_view = [&](int a) {
+        const T* src = raw + (int64_t)a * V * U;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) xn[h][k] = 0.f;
+            if (rok[h]) Oct<T>::load(src + (int64_t)(v0 + rv + 32 * h) * U + u0 + ru, xn[h]);
+        }
+    };
+    if (a0 < A) load_view(a0);
+    for (int j = 0; j < NV; ++j) {
+        const int a = a0 + j;
+        if (a >= A) break;                       // uniform over the workgroup
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float x = xn[h][k];
+                const float p = take_log ? -__logf(fmaxf((x - dd[h][k]) * gi[h][k], tmin)) : x;
+                tile[(ru + k) * PT_PAD + rv + 32 * h] = p;
+            }
+        }
+        __syncthreads();
+        if (j + 1 < NV && a + 1 < A) load_view(a + 1);
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int ul = wu + 16 * h, u = u0 + ul, v = v0 + wv;
+            if (u < U && v < V) {                // V % 4 == 0: a chunk is wholly inside or outside
+                const float* r = tile + ul * PT_PAD + wv;
+                *reinterpret_cast<float4*>(out + ((int64_t)a * U + u) * V + v) = make_float4(r[0], r[1], r[2], r[3]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 }  // namespace cbct
 
@@ -145,6 +236,18 @@ extern "C" int cbct_preprocess(const void* raw, int raw_dtype, const float* dark
     hipStream_t s = (hipStream_t)stream;
     const int R = (int)rows, Cc = (int)cols, lg = take_log ? 1 : 0;
     const size_t elem = raw_dtype == 0 ? 2 : 4;
+    if (transpose && cols % 8 == 0 && rows % 4 == 0 && ((uintptr_t)raw % 16) == 0 && ((uintptr_t)out % 16) == 0) {
+        constexpr int NV = 8;
+        const dim3 vgrid((unsigned)tiles, (unsigned)ceil_div(A, (int64_t)NV));
+        if (raw_dtype == 0)
+            preprocess_views_kernel<uint16_t, NV><<<vgrid, 256, 0, s>>>((const uint16_t*)raw, dark, flat, (int)A, R, Cc,
+                                                                       min_transmission, lg, out);
+        else
+            preprocess_views_kernel<float, NV><<<vgrid, 256, 0, s>>>((const float*)raw, dark, flat, (int)A, R, Cc,
+                                                                    min_transmission, lg, out);
+        CBCT_LAUNCHED("preprocess kernel launch");
+        return CBCT_OK;
+    }
     if (transpose && rows % 2 == 0 && cols % 2 == 0 && ((uintptr_t)raw % (2 * elem)) == 0 &&
         ((uintptr_t)out % 8) == 0) {
         const dim3 pgrid((unsigned)(ceil_div(rows, (int64_t)PT) * ceil_div(cols, (int64_t)PW)), (unsigned)A);

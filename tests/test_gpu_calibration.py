@@ -20,6 +20,9 @@ DEV = torch.device("cuda:0")
 
 SMALL = {"D": 40, "A": 18, "U": 70, "V": 50, "voxel": 0.5, "sod": 600.0, "sid": 1100.0, "seed": 5}
 SMALL["pixel"] = (1100 / 600) * 1.42 * 40 * 0.5 / (0.95 * 50)
+# a voxel size that is not a power of two: the fused march takes the forward's division map
+SMALL_DIV = dict(SMALL, voxel=0.8, seed=6)
+SMALL_DIV["pixel"] = (1100 / 600) * 1.42 * 40 * 0.8 / (0.95 * 50)
 
 
 def _problem(spec, **kw):
@@ -43,13 +46,13 @@ def _rel(a, b):
     return ((a.double() - b.double()).norm() / b.double().norm().clamp_min(1e-30)).item()
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_fused_gradient_equals_plain_operators(use_graph):
+@pytest.mark.parametrize("use_graph,spec", [(False, SMALL), (True, SMALL), (False, SMALL_DIV)])
+def test_fused_gradient_equals_plain_operators(use_graph, spec):
     """Zero steps: the loss and its geometry gradient at the start geometry.  The fused march
     weights each ray's geometry sums by its residual once at the end, the plain adjoint weights
     every sample; the angle gradient sums terms of both signs over the view, so the two FP32
     orders are compared through their distance to the FP64 oracle."""
-    p = _problem(SMALL)
+    p = _problem(spec)
     g0 = p.geometry.clone()
     calibration.calibrate([p], steps=0, use_graph=use_graph, want_grad=True)
     assert torch.equal(p.geometry, g0)                      # no update at zero steps

@@ -10,7 +10,8 @@
 // per-view constants are wave-uniform and read through the scalar cache.
 //
 // Arithmetic follows csrc/src_cbct_backprojector_cuda.cu:37-95 (forward) and
-// :132-292 (backward) in the FP32 operation order fixed here and in the oracle.
+// :132-292 (backward); the detector coordinates use the reciprocal pixel size (one FMA instead of
+// a division per voxel-view), which differs from the reference's division by FP32 rounding.
 #include "cbct_common.h"
 
 #include <algorithm>
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_forward_kernel(
         const float uph = fmaf(t, px - f.dsu, f.dsu);
         const float sw = m / q;
         const float gw = sw * sw;
-        const float ui = uph / g.pix + f.u0;
+        const float ui = fmaf(uph, g.inv_pix, f.u0);
         if (!(ui >= 0.0f && ui < Um1)) continue;
         const int iu = (int)floorf(ui);
         const float fu = ui - (float)iu, mu = 1.0f - fu;
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_forward_kernel(
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const float vph = fmaf(t, vz[k] - f.dsv, f.dsv);
-            const float vi = vph / g.pix + f.v0;
+            const float vi = fmaf(vph, g.inv_pix, f.v0);
             if (vi >= 0.0f && vi < Vm1) {
                 const int iv = (int)floorf(vi);
                 const float fv = vi - (float)iv, mv = 1.0f - fv;

@@ -60,9 +60,10 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_forward_kernel(
         const float q = m - py;
         const float l = f.sid + f.dsn;
         if (q <= 1e-6f || l <= 1e-6f) continue;
-        const float t = l / q;
+        const float iq = 1.0f / q;
+        const float t = l * iq;
         const float uph = fmaf(t, px - f.dsu, f.dsu);
-        const float sw = m / q;
+        const float sw = m * iq;
         const float gw = sw * sw;
         const float ui = fmaf(uph, g.inv_pix, f.u0);
         if (!(ui >= 0.0f && ui < Um1)) continue;

@@ -75,7 +75,7 @@ def parse():
     p.add_argument("--train-steps", type=int, default=None,
                    help="training steps of the warm-start network in the calibration-time workload")
     p.add_argument("--deterministic", action="store_true",
-                   help="bitwise-reproducible volume gradient (int64 fixed-point accumulation)")
+                   help="bitwise-reproducible volume / projection gradients (int64 fixed-point accumulation)")
     p.add_argument("--scans", type=int, default=None,
                    help="held-out scans of the calibration-time workload (default 4)")
     p.add_argument("--fp-tuning", default=None,
@@ -448,6 +448,8 @@ def backprojection_bench(args, torch, device):
     eight geometry gradients) for a fixed upstream volume gradient.  Algorithmic bytes per
     voxel-view: forward 16 (four taps); backward 16 (taps) + 16 (four scattered adds)."""
     from cbct_backprojector import _native
+    if args.deterministic:
+        _native.set_deterministic(True)
     w = WORKLOADS[BP_WORKLOAD]
     _, geom_np, vol, geom, _ = make_inputs(w, device)
     D, A, U, V = w["D"], w["A"], w["U"], w["V"]
@@ -487,7 +489,7 @@ def backprojection_bench(args, torch, device):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": BP_WORKLOAD, "volume": [D] * 3, "views": A, "detector": [U, V],
-                   "t_bp_ms": t_bp * 1e3, "t_bp_backward_ms": t_bb * 1e3},
+                   "t_bp_ms": t_bp * 1e3, "t_bp_backward_ms": t_bb * 1e3, "deterministic": args.deterministic},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
     }

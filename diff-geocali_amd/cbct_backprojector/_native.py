@@ -46,6 +46,7 @@ _SIGNATURES = {
     "cbct_fp_set_tuning_op": ([_INT, _INT, _INT, _INT, _INT], _INT),
     "cbct_fp_get_tuning_op": ([_INT, _P], _INT),
     "cbct_bp_workspace_bytes": ([_I64, _I64, _I64, _I64], _SZ),
+    "cbct_bp_backward_workspace_bytes": ([_I64] * 6, _SZ),
     "cbct_bp_forward": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
                         _P, _P, _SZ, _P], _INT),
     "cbct_bp_backward": ([_P, _P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
@@ -273,7 +274,7 @@ def bp_backward(grad_vol, proj, geometry, pixel_size, voxel_size, off_x, off_y, 
     with torch.cuda.device(dev):
         gproj = torch.empty((A, U, V), dtype=torch.float32, device=dev)
         ggeo = torch.empty((8, A), dtype=torch.float32, device=dev)
-        ws_bytes = L.cbct_bp_workspace_bytes(A, D, H, W)
+        ws_bytes = L.cbct_bp_backward_workspace_bytes(A, U, V, D, H, W)
         ws = _workspace(ws_bytes, dev)
         _check(L.cbct_bp_backward(_ptr(grad_vol), _ptr(proj), A, U, V, *[_ptr(g) for g in geometry],
                                   pixel_size, D, H, W, voxel_size, off_x, off_y, off_z,

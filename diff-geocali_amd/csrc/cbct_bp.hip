@@ -112,6 +112,14 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_forward_kernel(
 // global atomics than four per voxel-view.  A window larger than the tile falls back to direct
 // global atomics for that view.  The fixed-point scale 2^e per view keeps the largest possible
 // cell total (max|grad| x max weight x columns x voxels of a column per cell) below 2^30.
+//
+// The eight geometry sums of a view are combined over the workgroup's waves in a fixed order
+// and stored as the workgroup's partial (no atomics); launch_reduce_geometry sums the partials
+// of each view in a fixed order, so grad_geometry is bitwise reproducible.  With DET the
+// projection gradient is accumulated as int64 fixed point at a per-view scale 2^E[a] (E from
+// max|grad_volume| x the view's largest weight, so one sample stays below 2^40 units): the LDS
+// cells are converted exactly from 2^e to 2^E[a] (rounded when e > E[a]) and added with int64
+// atomics, whose sum does not depend on their order.
 constexpr int BT = 8;                          // column block edge
 constexpr int BT_COLS_PER_WAVE = BT * BT / BP_WAVES;
 constexpr int BT_CAP = 8192;                   // LDS tile cells
@@ -146,14 +154,17 @@ __device__ __forceinline__ float bp_vi(const ViewFrame& f, float t, float vz, fl
     return fmaf(vph, ip, f.v0);
 }
 
-template <int K>
+template <int K, bool DET>
 __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
     const float* __restrict__ gvol, const float* __restrict__ proj,
     const ViewFrame* __restrict__ frames, BpGeom g,
-    float* __restrict__ gproj, float* __restrict__ ggeom) {
+    float* __restrict__ gproj, float* __restrict__ partials,
+    unsigned long long* __restrict__ gacc, const int* __restrict__ gexp) {
     __shared__ unsigned tile[BT_CAP];
     __shared__ float red[BP_WAVES];
+    __shared__ float gred[BP_WAVES][8];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int blk = blockIdx.y * gridDim.x + blockIdx.x, nblk = gridDim.x * gridDim.y;
     const int nbx = (g.W + BT - 1) / BT;
     const int bx = blockIdx.x % nbx, by = blockIdx.x / nbx;
     const int z0 = blockIdx.y * 64 * K;
@@ -212,17 +223,22 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
             gwmax = fmaxf(gwmax, __shfl_xor(gwmax, s, 64));
             tmin = fminf(tmin, __shfl_xor(tmin, s, 64));
         }
-        if (!(umax >= 0.f && umin < Um1 && vmax >= 0.f && vmin < Vm1)) continue;   // no tap on the detector
+        float* part = partials + ((int64_t)a * nblk + blk) * 8;
+        if (!(umax >= 0.f && umin < Um1 && vmax >= 0.f && vmin < Vm1)) {      // no tap on the detector
+            if (threadIdx.x < 8) part[threadIdx.x] = 0.f;
+            continue;
+        }
+        const int Ea = DET ? gexp[a] : 0;
         const int u_lo = (int)fmaxf(floorf(umin), 0.f), u_hi = (int)fminf(floorf(umax), (float)(g.U - 2));
         const int v_lo = (int)fmaxf(floorf(vmin), 0.f), v_hi = (int)fminf(floorf(vmax), (float)(g.V - 2));
         const int nu = u_hi - u_lo + 2, nv = v_hi - v_lo + 2;
         const bool staged = nu * nv <= BT_CAP;
         float scale = 1.f, inv_scale = 1.f;
+        int e = 0;
         if (staged) {
             const float spacing = g.vs * tmin * ip;                  // detector rows per voxel along z
             const float nz = spacing > 0.f ? fminf(floorf(2.0f / spacing) + 2.0f, 64.0f * K) : 64.0f * K;
             const float bound = amax * gwmax * (float)(BT * BT) * nz;
-            int e = 0;
             if (bound > 0.f) {
                 int k2;
                 frexpf(1073741824.f / bound, &k2);
@@ -292,6 +308,12 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
                         atomicAdd(q0 + nv, (unsigned)__float2int_rn(gb * fu * mv * scale));
                         atomicAdd(q0 + 1, (unsigned)__float2int_rn(gb * mu * fv * scale));
                         atomicAdd(q0 + nv + 1, (unsigned)__float2int_rn(gb * fu * fv * scale));
+                    } else if (DET) {
+                        atomicAdd(gacc + k00, (unsigned long long)__double2ll_rn(ldexp((double)(gb * mu * mv), Ea)));
+                        atomicAdd(gacc + k00 + g.V, (unsigned long long)__double2ll_rn(ldexp((double)(gb * fu * mv), Ea)));
+                        atomicAdd(gacc + k00 + 1, (unsigned long long)__double2ll_rn(ldexp((double)(gb * mu * fv), Ea)));
+                        atomicAdd(gacc + k00 + g.V + 1,
+                                  (unsigned long long)__double2ll_rn(ldexp((double)(gb * fu * fv), Ea)));
                     } else {
                         atomicAdd(gproj + k00, gb * mu * mv);
                         atomicAdd(gproj + k00 + g.V, gb * fu * mv);
@@ -322,13 +344,24 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
         s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
         s4 = wave_sum(s4); s5 = wave_sum(s5); s6 = wave_sum(s6); s7 = wave_sum(s7);
         if (lane == 0) {
-            atomicAdd(ggeom + 0 * g.A + a, s0); atomicAdd(ggeom + 1 * g.A + a, s1);
-            atomicAdd(ggeom + 2 * g.A + a, s2); atomicAdd(ggeom + 3 * g.A + a, s3);
-            atomicAdd(ggeom + 4 * g.A + a, s4); atomicAdd(ggeom + 5 * g.A + a, s5);
-            atomicAdd(ggeom + 6 * g.A + a, s6); atomicAdd(ggeom + 7 * g.A + a, s7);
+            gred[wave][0] = s0; gred[wave][1] = s1; gred[wave][2] = s2; gred[wave][3] = s3;
+            gred[wave][4] = s4; gred[wave][5] = s5; gred[wave][6] = s6; gred[wave][7] = s7;
         }
         __syncthreads();
-        if (staged) {
+        if (threadIdx.x < 8) {
+            const int k = threadIdx.x;
+            part[k] = (gred[0][k] + gred[1][k]) + (gred[2][k] + gred[3][k]);
+        }
+        if (staged && DET) {
+            const int sh = Ea - e;      // cell units 2^-e -> accumulator units 2^-Ea
+            for (int r = 0; r < nu; ++r) {
+                unsigned long long* dst = gacc + ((int64_t)a * g.U + u_lo + r) * g.V + v_lo;
+                for (int cc = threadIdx.x; cc < nv; cc += 64 * BP_WAVES) {
+                    const int iv = (int)tile[r * nv + cc];
+                    if (iv != 0) atomicAdd(dst + cc, (unsigned long long)__double2ll_rn(ldexp((double)iv, sh)));
+                }
+            }
+        } else if (staged) {
             for (int r = 0; r < nu; ++r) {
                 float* dst = gproj + ((int64_t)a * g.U + u_lo + r) * g.V + v_lo;
                 for (int cc = threadIdx.x; cc < nv; cc += 64 * BP_WAVES) {
@@ -339,6 +372,47 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
         }
         __syncthreads();
     }
+}
+
+static_assert(BP_WAVES == 4, "bp_backward_kernel combines the geometry sums of four waves");
+
+// Per-view accumulator exponent of the deterministic backward: E[a] = 40 - k where
+// max|grad_volume| x (largest weight of a column whose u lands on the detector) = f 2^k,
+// f in [0.5, 1), so that one sample's contribution stays below 2^40 units of 2^-E[a].
+__global__ __launch_bounds__(256) void bp_det_exponent_kernel(const ViewFrame* __restrict__ frames, BpGeom g,
+                                                              const unsigned* __restrict__ maxbits,
+                                                              int* __restrict__ gexp) {
+    __shared__ float red[4];
+    const int a = blockIdx.x;
+    const ViewFrame f = frames[a];
+    const float Um1 = (float)g.U - 1.0f;
+    float wmax = 0.f;
+    for (int i = threadIdx.x; i < g.H * g.W; i += 256) {
+        const int x = i % g.W, y = i / g.W;
+        const BpColumn c = bp_column(f, fmaf((float)x - g.hx, g.vs, g.offx), fmaf((float)y - g.hy, g.vs, g.offy),
+                                     g.inv_pix);
+        if (c.ok && c.ui >= 0.0f && c.ui < Um1) wmax = fmaxf(wmax, c.gw);
+    }
+    for (int s = 32; s >= 1; s >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, s, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float m = __uint_as_float(maxbits[0]) * fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        int E = 0;
+        if (m > 0.f && isfinite(m)) {
+            int k;
+            frexpf(m, &k);
+            E = max(-60, min(120, 40 - k));
+        }
+        gexp[a] = E;
+    }
+}
+
+__global__ void bp_det_convert_kernel(const unsigned long long* __restrict__ acc, int64_t n, int64_t per_view,
+                                      const int* __restrict__ gexp, float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = (float)ldexp((double)(long long)acc[i], -gexp[i / per_view]);
 }
 
 static int check_bp_common(int64_t A, int64_t U, int64_t V, int64_t D, int64_t H, int64_t W,
@@ -369,6 +443,14 @@ static BpGeom make_bp_geom(int64_t A, int64_t U, int64_t V, int64_t D, int64_t H
 
 static int bp_k(int64_t D) { return D > 128 ? 4 : (D > 64 ? 2 : 1); }
 
+// Backward launch shape: 8x8 columns x 64*K voxels per workgroup.
+static int bp_bwd_k(int64_t D) { return bp_k(D) >= 2 ? 2 : 1; }
+static dim3 bp_bwd_grid(int64_t D, int64_t H, int64_t W) {
+    return dim3((unsigned)(ceil_div(W, (int64_t)BT) * ceil_div(H, (int64_t)BT)),
+                (unsigned)ceil_div(D, 64 * bp_bwd_k(D)));
+}
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
 int launch_bp_forward(const float* proj, const ViewFrame* frames, const BpGeom& g, int chunk,
                       float scale, int clamp, float* vol, hipStream_t stream) {
     const int K = bp_k(g.D);
@@ -398,6 +480,18 @@ extern "C" size_t cbct_bp_workspace_bytes(int64_t A, int64_t D, int64_t H, int64
     (void)D; (void)H; (void)W;
     if (A < 1) return 0;
     return (size_t)A * sizeof(ViewFrame);
+}
+
+// frames | per-workgroup geometry partials [A][blocks][8] | (deterministic) int64 accumulator
+// [A,U,V] | max bits + per-view exponents
+extern "C" size_t cbct_bp_backward_workspace_bytes(int64_t A, int64_t U, int64_t V, int64_t D, int64_t H,
+                                                   int64_t W) {
+    if (A < 1 || U < 1 || V < 1 || D < 1 || H < 1 || W < 1) return 0;
+    const dim3 grid = bp_bwd_grid(D, H, W);
+    size_t n = align256((size_t)A * sizeof(ViewFrame)) + align256((size_t)A * grid.x * grid.y * 8 * sizeof(float));
+    if (cbct_fp_get_deterministic())
+        n += align256((size_t)A * U * V * sizeof(unsigned long long)) + align256(sizeof(unsigned) * (1 + A));
+    return n;
 }
 
 extern "C" int cbct_bp_forward(const float* projections, int64_t A, int64_t U, int64_t V,
@@ -437,23 +531,52 @@ extern "C" int cbct_bp_backward(const float* grad_volume, const float* projectio
     if ((st = check_geometry(gg, A))) return st;
     CBCT_CHECK_ARG(grad_volume != nullptr && projections != nullptr && grad_projections != nullptr &&
                    grad_geometry != nullptr, "gradient and projection pointers must be non-null");
-    CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= cbct_bp_workspace_bytes(A, D, H, W),
-                   "workspace too small for cbct_bp_backward");
+    CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= cbct_bp_backward_workspace_bytes(A, U, V, D, H, W),
+                   "workspace too small for cbct_bp_backward (see cbct_bp_backward_workspace_bytes)");
+    const bool det = cbct_fp_get_deterministic() != 0;
     hipStream_t stream = (hipStream_t)stream_;
+    const dim3 grid = bp_bwd_grid(D, H, W);
+    const int64_t nblk = (int64_t)grid.x * grid.y;
     ViewFrame* frames = (ViewFrame*)workspace;
+    float* partials = (float*)((char*)workspace + align256((size_t)A * sizeof(ViewFrame)));
     const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
     if ((st = launch_frames(gp, A, frames, stream))) return st;
-    if ((st = launch_fill(grad_projections, A * U * V, 0.0f, stream))) return st;
-    if ((st = launch_fill(grad_geometry, 8 * A, 0.0f, stream))) return st;
     const BpGeom g = make_bp_geom(A, U, V, D, H, W, pixel_size, voxel_size, off_x, off_y, off_z);
-    const int K = bp_k(D) >= 2 ? 2 : 1;      // 8x8 columns x 64K voxels per workgroup
-    dim3 grid((unsigned)(ceil_div(W, (int64_t)BT) * ceil_div(H, (int64_t)BT)), (unsigned)ceil_div(D, 64 * K));
-    if (K == 2)
-        bp_backward_kernel<2><<<grid, 64 * BP_WAVES, 0, stream>>>(grad_volume, projections, frames, g,
-                                                                 grad_projections, grad_geometry);
-    else
-        bp_backward_kernel<1><<<grid, 64 * BP_WAVES, 0, stream>>>(grad_volume, projections, frames, g,
-                                                                 grad_projections, grad_geometry);
+    unsigned long long* gacc = nullptr;
+    int* gexp = nullptr;
+    const int64_t nproj = A * U * V;
+    if (det) {
+        gacc = (unsigned long long*)((char*)partials + align256((size_t)A * nblk * 8 * sizeof(float)));
+        unsigned* maxbits = (unsigned*)((char*)gacc + align256((size_t)nproj * sizeof(unsigned long long)));
+        gexp = (int*)(maxbits + 1);
+        if ((st = launch_zero_u64(gacc, nproj, stream))) return st;
+        if ((st = launch_fill((float*)maxbits, 1, 0.0f, stream))) return st;
+        if ((st = launch_absmax(grad_volume, D * H * W, maxbits, stream))) return st;
+        bp_det_exponent_kernel<<<(unsigned)A, 256, 0, stream>>>(frames, g, maxbits, gexp);
+        CBCT_LAUNCHED("deterministic backward exponent kernel");
+    } else if ((st = launch_fill(grad_projections, nproj, 0.0f, stream))) {
+        return st;
+    }
+    if (bp_bwd_k(D) == 2) {
+        if (det)
+            bp_backward_kernel<2, true><<<grid, 64 * BP_WAVES, 0, stream>>>(grad_volume, projections, frames, g,
+                                                                           grad_projections, partials, gacc, gexp);
+        else
+            bp_backward_kernel<2, false><<<grid, 64 * BP_WAVES, 0, stream>>>(grad_volume, projections, frames, g,
+                                                                            grad_projections, partials, gacc, gexp);
+    } else {
+        if (det)
+            bp_backward_kernel<1, true><<<grid, 64 * BP_WAVES, 0, stream>>>(grad_volume, projections, frames, g,
+                                                                           grad_projections, partials, gacc, gexp);
+        else
+            bp_backward_kernel<1, false><<<grid, 64 * BP_WAVES, 0, stream>>>(grad_volume, projections, frames, g,
+                                                                            grad_projections, partials, gacc, gexp);
+    }
     CBCT_LAUNCHED("backprojection backward kernel launch");
-    return CBCT_OK;
+    if (det) {
+        bp_det_convert_kernel<<<(unsigned)ceil_div(nproj, (int64_t)256), 256, 0, stream>>>(gacc, nproj, U * V, gexp,
+                                                                                         grad_projections);
+        CBCT_LAUNCHED("deterministic projection gradient conversion");
+    }
+    return launch_reduce_geometry(partials, A, nblk, grad_geometry, stream);
 }

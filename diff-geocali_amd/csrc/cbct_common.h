@@ -291,5 +291,11 @@ __global__ void frames_kernel(GeomPtrs g, int A, ViewFrame* __restrict__ out);
 __global__ void fill_kernel(float* __restrict__ p, int64_t n, float value);
 int launch_frames(const GeomPtrs& g, int64_t A, ViewFrame* out, hipStream_t stream);
 int launch_fill(float* p, int64_t n, float value, hipStream_t stream);
+// Kernels of cbct_fp.hip shared with the backprojector backward: zero n int64 words; max |x|
+// over n floats into *maxbits (as float bits, *maxbits cleared beforehand); ggeom[k][a] = the
+// fixed-order sum of partials[(a * tiles + t) * 8 + k] over t.
+int launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t stream);
+int launch_absmax(const float* x, int64_t n, unsigned* maxbits, hipStream_t stream);
+int launch_reduce_geometry(const float* partials, int64_t A, int64_t tiles, float* ggeom, hipStream_t stream);
 
 }  // namespace cbct

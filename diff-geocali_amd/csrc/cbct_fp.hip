@@ -231,14 +231,28 @@ __global__ void zero_u64_kernel(unsigned long long* __restrict__ p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
 }
 
-// max |x| over n floats as the bit pattern of a non-negative float (order-independent max)
+// max |x| over n floats as the bit pattern of a non-negative float (order-independent max);
+// one atomic per workgroup, 16-byte loads when x is 16-byte aligned.
 __global__ __launch_bounds__(256) void absmax_kernel(const float* __restrict__ x, int64_t n,
                                                      unsigned* __restrict__ maxbits) {
+    __shared__ float red[4];
     float m = 0.f;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) m = fmaxf(m, fabsf(x[i]));
+    const int64_t first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n4 = ((uintptr_t)x & 15) == 0 ? n / 4 : 0;
+    const float4* __restrict__ x4 = reinterpret_cast<const float4*>(x);
+    for (int64_t i = first; i < n4; i += stride) {
+        const float4 v = x4[i];
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+    }
+    for (int64_t i = 4 * n4 + first; i < n; i += stride) m = fmaxf(m, fabsf(x[i]));
     for (int s = 32; s >= 1; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
-    if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(maxbits, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        if (m > 0.f) atomicMax(maxbits, __float_as_uint(m));
+    }
 }
 
 // global exponent E: the largest per-sample contribution |g * step| < 2^40 units of 2^-E
@@ -258,6 +272,26 @@ __global__ void det_convert_kernel(const unsigned long long* __restrict__ acc, i
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     out[i] = (float)ldexp((double)(long long)acc[i], -gexp[0]);
+}
+
+int launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t stream) {
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, (int64_t)256), 16384));
+    zero_u64_kernel<<<blocks, 256, 0, stream>>>(p, n);
+    CBCT_LAUNCHED("int64 accumulator reset");
+    return CBCT_OK;
+}
+
+int launch_absmax(const float* x, int64_t n, unsigned* maxbits, hipStream_t stream) {
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, (int64_t)256 * 16), 2048));
+    absmax_kernel<<<blocks, 256, 0, stream>>>(x, n, maxbits);
+    CBCT_LAUNCHED("max |x| kernel");
+    return CBCT_OK;
+}
+
+int launch_reduce_geometry(const float* partials, int64_t A, int64_t tiles, float* ggeom, hipStream_t stream) {
+    fp_reduce_geometry<<<(unsigned)A, 256, 0, stream>>>(partials, (int)A, (int)tiles, ggeom);
+    CBCT_LAUNCHED("geometry reduction kernel launch");
+    return CBCT_OK;
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -392,8 +426,7 @@ extern "C" int cbct_fp_backward(const float* grad_projection, const float* volum
         zero_u64_kernel<<<1, 64, 0, stream>>>((unsigned long long*)maxbits, 1);
         CBCT_LAUNCHED("deterministic accumulator reset");
         const int64_t ng = A * U * V;
-        absmax_kernel<<<(unsigned)std::min<int64_t>(ceil_div(ng, (int64_t)256), 8192), 256, 0, stream>>>(
-            grad_projection, ng, maxbits);
+        if ((st = launch_absmax(grad_projection, ng, maxbits, stream))) return st;
         det_exponent_kernel<<<1, 1, 0, stream>>>(maxbits, step_size, gexp);
         CBCT_LAUNCHED("deterministic scale kernels");
     } else if (compute_grad_volume && (st = launch_fill(grad_volume, nvox, 0.0f, stream))) {

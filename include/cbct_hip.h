@@ -65,7 +65,8 @@ size_t cbct_fp_backward_workspace_bytes(int64_t D, int64_t H, int64_t W, int64_t
  * gradient is accumulated as int64 fixed-point values (scale 2^-E with E chosen
  * from max|grad * step|, so each sample keeps >= 40 bits), which makes the result
  * bitwise reproducible run to run; the reference's float atomics are not
- * (docs/USAGE.md:306).  Needs the slab kernels (use_slab = 1). */
+ * (docs/USAGE.md:306).  Needs the slab kernels (use_slab = 1).  The same setting
+ * makes cbct_bp_backward's projection gradient reproducible. */
 int cbct_fp_set_deterministic(int on);
 int cbct_fp_get_deterministic(void);
 
@@ -117,8 +118,14 @@ int cbct_fp_get_tuning_op(int op, int* out4);
 
 /* ---- weighted backprojector: src_cbct_backprojector_cuda --------------- */
 
-/* Workspace bytes needed by cbct_bp_forward / cbct_bp_backward. */
+/* Workspace bytes needed by cbct_bp_forward. */
 size_t cbct_bp_workspace_bytes(int64_t A, int64_t D, int64_t H, int64_t W);
+
+/* Workspace bytes needed by cbct_bp_backward: the view frames, one 8-float
+ * geometry partial per workgroup and view, and, in deterministic mode
+ * (cbct_fp_set_deterministic), 8 bytes per detector cell for the integer
+ * accumulator.  Query it with the deterministic setting that the call will use. */
+size_t cbct_bp_backward_workspace_bytes(int64_t A, int64_t U, int64_t V, int64_t D, int64_t H, int64_t W);
 
 /* volume[D,H,W] = sum over views of FDK-weighted bilinear samples of
  * projections[A,U,V].  Replaces forward_cuda,
@@ -133,7 +140,11 @@ int cbct_bp_forward(const float* projections, int64_t A, int64_t U, int64_t V,
 
 /* Backward of cbct_bp_forward for grad_volume[D,H,W]: grad_projections[A,U,V]
  * and grad_geometry[8][A].  Replaces backward_cuda,
- * csrc/src_cbct_backprojector_cuda.cu:423-522 (pybind "backward", :534-542). */
+ * csrc/src_cbct_backprojector_cuda.cu:423-522 (pybind "backward", :534-542).
+ * grad_geometry is summed in a fixed order (bitwise reproducible); in
+ * deterministic mode grad_projections is accumulated as int64 fixed point at a
+ * per-view scale 2^-E (one sample < 2^40 units) and is bitwise reproducible too;
+ * otherwise its float atomics make the last bits vary run to run. */
 int cbct_bp_backward(const float* grad_volume, const float* projections,
                      int64_t A, int64_t U, int64_t V,
                      const float* angles, const float* sid, const float* sod,

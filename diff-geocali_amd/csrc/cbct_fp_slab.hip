@@ -138,8 +138,8 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     // operand (half) and needs no per-sample copy of the other.
     float inv_vs_v = P.inv_vs;
     asm volatile("" : "+v"(inv_vs_v));
-    auto voxel_coords = [&](float& x, float& y, float& z) {
-        const float px = fmaf(t, ray.dx, f.ox), py = fmaf(t, ray.dy, f.oy), pz = fmaf(t, ray.dz, f.oz);
+    auto voxel_coords_at = [&](float tt, float& x, float& y, float& z) {
+        const float px = fmaf(tt, ray.dx, f.ox), py = fmaf(tt, ray.dy, f.oy), pz = fmaf(tt, ray.dz, f.oz);
         // ZOFF: all volume offsets are zero, and p - 0 == p exactly, so the subtraction folds away
         const float ox_ = ZOFF ? 0.f : b.offx, oy_ = ZOFF ? 0.f : b.offy, oz_ = ZOFF ? 0.f : b.offz;
         if (MODE == 0 || MODE >= 3) {      // the forward's voxel map (division unless vs is a power of two)
@@ -152,6 +152,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             z = fmaf(pz - oz_, inv_vs_v, b.hz);
         }
     };
+    auto voxel_coords = [&](float& x, float& y, float& z) { voxel_coords_at(t, x, y, z); };
     // Everything a sample does once its eight taps are known.  Lanes with go == false (no sample in
     // the current slab) leave every sum unchanged.
     auto use_taps = [&](const Taps& tp, float fx, float fy, float fz, bool go) {
@@ -314,6 +315,38 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         const int PL = B0 * B1;
         // largest base offset whose eight taps stay inside the staged box (that of the last cell)
         const unsigned safe = staged ? (unsigned)(B0 * B1 * B2 - PL - B0 - 2) : 0u;
+        // Per lane, once per slab: do the taps of every sample of this slab lie in the box?  The
+        // lane's samples here lie between its current t and t_end = min(t_x, tmax), where t_x is a
+        // t at which the principal coordinate has left the slab (solved for, moved on by 0.01
+        // voxel, and checked).  Each voxel coordinate is a monotone function of t (fmaf and the
+        // voxel map are correctly rounded, with fixed signs), so if the cells at both ends have
+        // their taps in the box, so do all samples in between, and no sample needs its own test.
+        // A lane that fails takes the global path for this slab.
+        bool lane_safe = false;
+        if (staged && t <= tmax) {
+            float x, y, z;
+            voxel_coords(x, y, z);
+            bool ok = (unsigned)(floor_int(x) - lo[0]) < bx1 && (unsigned)(floor_int(y) - lo[1]) < by1 &&
+                      (unsigned)(floor_int(z) - lo[2]) < bz1;
+            const float dp = PAX == 0 ? ray.dx : (PAX == 1 ? ray.dy : ray.dz);
+            const float op = PAX == 0 ? f.ox : (PAX == 1 ? f.oy : f.oz);
+            const float offp = PAX == 0 ? b.offx : (PAX == 1 ? b.offy : b.offz);
+            const float hp = PAX == 0 ? b.hx : (PAX == 1 ? b.hy : b.hz);
+            const float target = (float)(FWD ? s_hi + 1 : s_lo);
+            const float tx = (fmaf(target - hp, P.vs, offp) - op) / dp + 0.01f * P.vs / fabsf(dp);
+            float te = tmax;
+            if (tx < tmax) {
+                te = tx;
+                voxel_coords_at(tx, x, y, z);
+                const float cp = PAX == 0 ? x : (PAX == 1 ? y : z);
+                ok = ok && (FWD ? cp >= target : cp < target);
+            }
+            voxel_coords_at(te, x, y, z);
+            ok = ok && (PAX == 0 || (unsigned)(floor_int(x) - lo[0]) < bx1) &&
+                 (PAX == 1 || (unsigned)(floor_int(y) - lo[1]) < by1) &&
+                 (PAX == 2 || (unsigned)(floor_int(z) - lo[2]) < bz1);
+            lane_safe = ok;
+        }
         for (;;) {
             float x, y, z;
             voxel_coords(x, y, z);
@@ -328,7 +361,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             if (m_go == 0ull) break;
             const unsigned ox = (unsigned)(xr - lo[0]), oy = (unsigned)(yr - lo[1]), oz = (unsigned)(zr - lo[2]);
             const bool go = live && ins;
-            const bool gl = go && ox < bx1 && oy < by1 && oz < bz1;
+            const bool gl = go && lane_safe;
             const float fx = frac(x), fy = frac(y), fz = frac(z);
             if (staged) {
                 // 24-bit unsigned products (box offsets below the LDS capacity): full-rate multiplies.

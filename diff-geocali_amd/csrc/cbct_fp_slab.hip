@@ -16,10 +16,10 @@
 // The arithmetic of every sample is the same float code as the direct kernels
 // (same t schedule, same cells, same interpolation), so projections and geometry
 // gradients are bit-identical to them; only the order of the volume-gradient
-// atomics differs.  A sample whose taps fall outside the staged box (never in
-// a regular cone-beam geometry) and tiles whose rays do not share one principal
-// direction take the direct path from global memory, so the staging only ever
-// affects speed.
+// atomics differs.  A ray that cannot show that its samples of a slab have their
+// taps in the staged box (never in a regular cone-beam geometry) and tiles whose
+// rays do not share one principal direction take the direct path from global
+// memory, so the staging only ever affects speed.
 #include "cbct_fp_slab.h"
 
 #include <algorithm>
@@ -303,9 +303,9 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     // the taps read from the staged box.  PAX / FWD (the tile's principal axis and direction) are
     // compile-time constants, so the slab-exit test is one compare.  The loop runs while any lane
     // of the wave has a sample in the slab; a lane without one rides along with its updates masked
-    // (go == false), so the loop has no divergent exits.  A sample whose taps leave the box (not
-    // seen in regular cone-beam geometry; every sample when the slab is not staged) reads them
-    // from global memory.
+    // (go == false), so the loop has no divergent exits.  A lane that cannot show that all its
+    // samples of the slab have their taps in the box (not seen in regular cone-beam geometry;
+    // every lane when the slab is not staged) reads them from global memory.
     auto march = [&](auto pax_c, auto fwd_c, int s_lo, int s_hi, const int* lo, int B0, int B1, int B2,
                      bool staged) {
         constexpr int PAX = decltype(pax_c)::value;

@@ -313,8 +313,11 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         const unsigned bx1 = staged ? (unsigned)(B0 - 1) : 0u, by1 = staged ? (unsigned)(B1 - 1) : 0u;
         const unsigned bz1 = staged ? (unsigned)(B2 - 1) : 0u;
         const int PL = B0 * B1;
-        // largest base offset whose eight taps stay inside the staged box (that of the last cell)
-        const unsigned safe = staged ? (unsigned)(B0 * B1 * B2 - PL - B0 - 2) : 0u;
+        // largest base offset whose eight taps stay inside the staged box (that of the last cell),
+        // in bytes, and the byte offset of the box's first cell in cell coordinates
+        const unsigned safe4 = staged ? 4u * (unsigned)(B0 * B1 * B2 - PL - B0 - 2) : 0u;
+        const int base4 = 4 * (lo[2] * PL + lo[1] * B0 + lo[0]);
+        const int B04 = 4 * B0, PL4 = 4 * PL;
         // Per lane, once per slab: do the taps of every sample of this slab lie in the box?  The
         // lane's samples here lie between its current t and t_end = min(t_x, tmax), where t_x is a
         // t at which the principal coordinate has left the slab (solved for, moved on by 0.01
@@ -359,16 +362,17 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             const bool ins = FWD ? fp <= s_hi : fp >= s_lo;
             const uint64_t m_go = __builtin_amdgcn_ballot_w64(live) & __builtin_amdgcn_ballot_w64(ins);
             if (m_go == 0ull) break;
-            const unsigned ox = (unsigned)(xr - lo[0]), oy = (unsigned)(yr - lo[1]), oz = (unsigned)(zr - lo[2]);
             const bool go = live && ins;
             const bool gl = go && lane_safe;
             const float fx = frac(x), fy = frac(y), fz = frac(z);
             if (staged) {
-                // 24-bit unsigned products (box offsets below the LDS capacity): full-rate multiplies.
-                // Lanes outside the box read a clamped in-box address and discard the result.
-                const unsigned raw = __umul24(oz, (unsigned)PL) + __umul24(oy, (unsigned)B0) + ox;
-                const int idx = (int)min(raw, safe);
-                const Taps tp = load_taps_strided(vbox, idx, 1, B0, PL);
+                // byte offset of the cell in the box: 24-bit products (cell coordinates and box
+                // strides are small), full rate.  Lanes outside the box read a clamped in-box
+                // address and discard the result.
+                const int raw4 = __mul24(zr, PL4) + __mul24(yr, B04) + 4 * xr - base4;
+                const unsigned idx4 = min((unsigned)raw4, safe4);
+                const Taps tp = load_taps_strided(reinterpret_cast<const float*>(
+                                                      reinterpret_cast<const char*>(vbox) + idx4), 0, 1, B0, PL);
                 use_taps(tp, fx, fy, fz, gl);
                 if (MODE == 2 && gl) {
                     // the scaled trilinear weights as a z, then y split of gsq, then x weights
@@ -377,7 +381,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                     const float g1 = gsq * fz, g0 = gsq - g1;
                     const float g01 = g0 * fy, g00 = g0 - g01, g11 = g1 * fy, g10 = g1 - g11;
                     const float mx = 1.0f - fx;
-                    unsigned* qi = gboxu + idx;
+                    unsigned* qi = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(gboxu) + idx4);
                     atomicAdd(qi, (unsigned)(int)fmaf(g00, mx, hq));
                     atomicAdd(qi + 1, (unsigned)(int)fmaf(g00, fx, hq));
                     atomicAdd(qi + B0, (unsigned)(int)fmaf(g01, mx, hq));

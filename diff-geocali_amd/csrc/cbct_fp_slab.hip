@@ -313,9 +313,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         const unsigned bx1 = staged ? (unsigned)(B0 - 1) : 0u, by1 = staged ? (unsigned)(B1 - 1) : 0u;
         const unsigned bz1 = staged ? (unsigned)(B2 - 1) : 0u;
         const int PL = B0 * B1;
-        // largest base offset whose eight taps stay inside the staged box (that of the last cell),
-        // in bytes, and the byte offset of the box's first cell in cell coordinates
-        const unsigned safe4 = staged ? 4u * (unsigned)(B0 * B1 * B2 - PL - B0 - 2) : 0u;
+        // byte offset of the box's first cell in cell coordinates
         const int base4 = 4 * (lo[2] * PL + lo[1] * B0 + lo[0]);
         const int B04 = 4 * B0, PL4 = 4 * PL;
         // Per lane, once per slab: do the taps of every sample of this slab lie in the box?  The
@@ -365,16 +363,15 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             const bool go = live && ins;
             const bool gl = go && lane_safe;
             const float fx = frac(x), fy = frac(y), fz = frac(z);
-            if (staged) {
-                // byte offset of the cell in the box: 24-bit products (cell coordinates and box
-                // strides are small), full rate.  Lanes outside the box read a clamped in-box
-                // address and discard the result.
-                const int raw4 = __mul24(zr, PL4) + __mul24(yr, B04) + 4 * xr - base4;
-                const unsigned idx4 = min((unsigned)raw4, safe4);
+            if (gl) {
+                // (gl: the slab is staged and the lane's samples in it are in the box; the other
+                // lanes skip the block.)  Byte offset of the cell in the box: 24-bit products (cell
+                // coordinates and box strides are small), full rate.
+                const unsigned idx4 = (unsigned)(__mul24(zr, PL4) + __mul24(yr, B04) + 4 * xr - base4);
                 const Taps tp = load_taps_strided(reinterpret_cast<const float*>(
                                                       reinterpret_cast<const char*>(vbox) + idx4), 0, 1, B0, PL);
-                use_taps(tp, fx, fy, fz, gl);
-                if (MODE == 2 && gl) {
+                use_taps(tp, fx, fy, fz, true);
+                if (MODE == 2) {
                     // the scaled trilinear weights as a z, then y split of gsq, then x weights
                     // plus half a unit of gsq's sign (every weight has it), so that truncation
                     // (one conversion) rounds to nearest, halves away from zero

@@ -52,8 +52,8 @@ DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 # Physical HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs of
 # `bench.py --pmc-probe` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB copy in the
 # same runs): adjoint fp_slab_kernel<2> at 256^3 x 720 views, 768x640 detector.
-MEASURED_TRAFFIC = {("geometry-gradients-256c-720v", "adjoint"): 105_984_128 * 1024 * 2.0 + 133_367_345 * 1024,
-                    ("geometry-gradients-256c-720v", "forward"): 112_893_839 * 1024 * 2.0 + 1_524_298 * 1024}
+MEASURED_TRAFFIC = {("geometry-gradients-256c-720v", "adjoint"): 116_375_994 * 1024 * 2.0 + 139_805_340 * 1024,
+                    ("geometry-gradients-256c-720v", "forward"): 113_387_769 * 1024 * 2.0 + 1_514_564 * 1024}
 PHANTOM_SEED = 20261203
 PERTURB_SEED = 7
 

@@ -398,7 +398,9 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         alive = t <= tmax;
     };
 
-    if (slab_ok) {
+    // (a tile none of whose rays meets the volume in slab order has nothing to stage)
+    const bool any_ray = __syncthreads_or(in_slabs ? 1 : 0) != 0;
+    if (slab_ok && any_ray) {
         // Slabs cover the principal floor coordinates -1 .. Np-1 in the rays' direction; a slab is
         // halved until its box fits the LDS capacity (uniform decision over the workgroup).
         int next = dirf > 0.f ? -1 : Np - 1;

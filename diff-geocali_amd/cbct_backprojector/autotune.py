@@ -28,7 +28,7 @@ SLAB_THICKNESS = (4, 8, 12, 16)
 LDS_FLOATS = (2048, 3840, 4096, 5120, 8192)
 LAYOUT = (0, 1)
 # The library's built-in settings per tuning slot (gathering kernels, volume-gradient adjoint).
-DEFAULT_GATHER = (16, 5120, 0)
+DEFAULT_GATHER = (12, 5120, 0)
 DEFAULT = DEFAULT_SCATTER = (12, 3840, 1)
 _SLOT = {"forward": 0, "geometry": 0, "calibration": 0, "adjoint": 1}
 

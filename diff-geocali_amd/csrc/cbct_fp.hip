@@ -222,7 +222,7 @@ struct FpTuning {
     int cap;              // LDS floats per staged box
     int layout;           // lane layout of a tile
 };
-static FpTuning g_tuning_op[2] = {{1, 16, 5120, 0}, {1, 12, 3840, 1}};
+static FpTuning g_tuning_op[2] = {{1, 12, 5120, 0}, {1, 12, 3840, 1}};
 static int g_deterministic = 0;   // volume gradient through order-independent int64 accumulation
 
 // ---- deterministic volume gradient helpers

@@ -408,7 +408,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     const bool any_ray = __syncthreads_or(in_slabs ? 1 : 0) != 0;
     if (slab_ok && any_ray) {
         // Slabs cover the principal floor coordinates -1 .. Np-1 in the rays' direction; a slab is
-        // halved until its box fits the LDS capacity (uniform decision over the workgroup).
+        // halved (to an even thickness) until its box fits the LDS capacity (uniform over the workgroup).
         int next = dirf > 0.f ? -1 : Np - 1;
         while (dirf > 0.f ? next <= Np - 1 : next >= -1) {
             int thick = P.T, s_lo, s_hi;
@@ -428,7 +428,10 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 if (!nonempty || thick == 1 ||
                     (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) <= P.cap)
                     break;
-                thick = (thick + 1) >> 1;
+                // halve, to an even thickness: the box's extent along the principal axis (the row
+                // pitch when that is x) is then odd, which keeps neighbouring lanes' taps on
+                // different LDS banks
+                thick = max(1, ((thick + 1) >> 1) & ~1);
             }
             next += dirf > 0.f ? thick : -thick;
             const bool empty = !nonempty;

@@ -296,9 +296,10 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             lo[k] = k == p ? lp : (k == o1 ? l1 : l2);
             hi[k] = k == p ? hp : (k == o1 ? h1 : h2);
         }
-        if (MODE == 2) {
+        if (MODE == 2 && !P.gacc) {
             // odd row and plane pitches for the adjoint's boxes: neighbouring lanes' eight
-            // atomics then spread over more LDS banks (171 vs 176 ms at 256^3 x 720)
+            // atomics then spread over more LDS banks (171 vs 176 ms at 256^3 x 720; not in
+            // deterministic mode, whose int64 flush of the larger boxes costs more: 207 vs 199 ms)
             hi[0] += (hi[0] - lo[0] + 1) & 1 ? 0 : 1;
             hi[1] += (hi[1] - lo[1] + 1) & 1 ? 0 : 1;
         }

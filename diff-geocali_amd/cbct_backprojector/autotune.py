@@ -43,6 +43,16 @@ def candidates():
     return list(itertools.product(SLAB_THICKNESS, LDS_FLOATS, LAYOUT))
 
 
+# The batched calibration launch gives every workgroup the largest LDS box of all its problems, so
+# one problem tuned to a large box lowers the occupancy of all of them: its tunings stay within
+# the box that keeps six workgroups (the batched kernel's register budget) on a CU.
+BATCH_MAX_LDS_FLOATS = 5120
+
+
+def batch_candidates():
+    return [c for c in candidates() if c[1] <= BATCH_MAX_LDS_FLOATS]
+
+
 def cache_path():
     return os.environ.get("CBCT_AUTOTUNE_CACHE",
                           os.path.join(os.path.expanduser("~"), ".cache", "cbct_hip", "autotune.json"))
@@ -141,7 +151,7 @@ def tune_calibration(problem, views=8, reps=2, use_cache=True, space=None):
     is also stored in ``problem.tuning``."""
     A, U, V = problem.measured.shape
     g = problem.geometry
-    key = shape_key("calibration", problem.volume.shape, U, V, problem.step_size, problem.voxel_size,
+    key = shape_key("calibration-batch", problem.volume.shape, U, V, problem.step_size, problem.voxel_size,
                     g[1].mean().item(), g[2].mean().item())
     hit = _load().get(key) if use_cache and space is None else None
     if hit:
@@ -152,7 +162,7 @@ def tune_calibration(problem, views=8, reps=2, use_cache=True, space=None):
                                          problem.pixel_size, problem.step_size, problem.voxel_size,
                                          problem.volume_offset)
     timings = {}
-    for c in (space or candidates()):
+    for c in (space or batch_candidates()):
         sub.tuning = c
         timings[c] = _median_ms(lambda: calibration.calibrate([sub], steps=0, use_graph=False), reps)
     best = min(timings, key=timings.get)

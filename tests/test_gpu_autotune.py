@@ -85,6 +85,6 @@ def test_calibration_tuner_sets_problem_tuning(tmp_path, monkeypatch):
             "pixel": 0.75}
     p = calibration.synthetic_problem(spec, DEV)
     best = autotune.tune_calibration(p, views=4, reps=1)
-    assert p.tuning == best and best in autotune.candidates()
+    assert p.tuning == best and best in autotune.batch_candidates()
     calibration.calibrate([p], steps=2)
     assert p.loss_history[2].item() < p.loss_history[0].item()

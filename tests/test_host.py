@@ -122,3 +122,6 @@ def test_autotune_cache_round_trip_and_shape_classes(tmp_path, monkeypatch):
     assert data[k1]["best"] == [12, 4096, 1] and data[k3]["best"] == [8, 8192, 0]
     assert len(autotune.candidates()) == 40 and autotune.DEFAULT in autotune.candidates()
     assert autotune.DEFAULT_GATHER in autotune.candidates()
+    batch = autotune.batch_candidates()
+    assert batch and set(batch) <= set(autotune.candidates())
+    assert max(c[1] for c in batch) == autotune.BATCH_MAX_LDS_FLOATS

@@ -45,7 +45,7 @@ def candidates():
 
 # The batched calibration launch gives every workgroup the largest LDS box of all its problems, so
 # one problem tuned to a large box lowers the occupancy of all of them: its tunings stay within
-# the box that keeps six workgroups (the batched kernel's register budget) on a CU.
+# the box that keeps seven workgroups (the batched kernel's register budget) on a CU.
 BATCH_MAX_LDS_FLOATS = 5120
 
 

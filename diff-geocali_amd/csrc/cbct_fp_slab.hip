@@ -664,7 +664,7 @@ void fp_slab_kernel(SlabArgs P) {
 // MODE 3 (Adam: loss gradient) or 4 (Levenberg-Marquardt: normal equations); one launch serves
 // one solver, so each mode is its own kernel with its own register budget.
 template <int MODE>
-__global__ __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void fp_slab_batched_kernel(const CalibDev* __restrict__ probs,
+__global__ __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(7))) void fp_slab_batched_kernel(const CalibDev* __restrict__ probs,
                                                                      int nprob, int total_tiles,
                                                                      unsigned* __restrict__ counter) {
     __shared__ int sh_tile;

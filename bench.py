@@ -53,7 +53,12 @@ DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 # `bench.py --pmc-probe` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB copy in the
 # same runs): adjoint fp_slab_kernel<2> at 256^3 x 720 views, 768x640 detector.
 MEASURED_TRAFFIC = {("geometry-gradients-256c-720v", "adjoint"): 128_418_581 * 1024 * 2.0 + 142_782_761 * 1024,
-                    ("geometry-gradients-256c-720v", "forward"): 113_586_044 * 1024 * 2.0 + 1_463_351 * 1024}
+                    ("geometry-gradients-256c-720v", "forward"): 113_586_044 * 1024 * 2.0 + 1_463_351 * 1024,
+                    # forward at 360 views, 512x384 (FDK round trip); backprojection backward at
+                    # 256^3 x 360 views (same counters, same factors)
+                    ("fdk-roundtrip-256c-360v", "forward"): 58_575_790 * 1024 * 2.0 + 295_300 * 1024,
+                    ("backprojection-256c-360v", "bp_backward"): 1_595_804 * 1024 * 2.0 + 6_993_152 * 1024,
+                    ("backprojection-256c-360v", "bp_forward"): 8_328_817 * 1024 * 2.0 + 172_207 * 1024}
 PHANTOM_SEED = 20261203
 PERTURB_SEED = 7
 
@@ -498,7 +503,8 @@ def backprojection_bench(args, torch, device):
         "config": {"workload": BP_WORKLOAD, "volume": [D] * 3, "views": A, "detector": [U, V],
                    "t_bp_ms": t_bp * 1e3, "t_bp_backward_ms": t_bb * 1e3, "deterministic": args.deterministic},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": None if args.deterministic else MEASURED_TRAFFIC.get((BP_WORKLOAD, dom[0]))},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_bp(w, geom_np, proj, gvol.cpu().numpy(), args.cpu_seconds)

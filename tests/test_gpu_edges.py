@@ -56,7 +56,7 @@ def _check_fp(vol, geom, pix, U, V, step, vs, off, seed):
                                    mode="f64")
     assert cases.rel_l2(gv, rvol) <= 1e-5
     # A geometry gradient is a sum with cancellation over few rays here, so its FP32 error is
-    # judged against the FP64 oracle: within 1e-4 of its size (the reference's own bar is 1e-3,
+    # compared with the FP64 oracle: within 1e-4 of its size (the reference's own bar is 1e-3,
     # examples/check_forward_operator.py:226) or at most twice the FP32 oracle's error.
     for k in range(8):
         e_gpu = np.abs(gg[k] - rgeo64[k]).max()

@@ -313,6 +313,10 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     // (go == false), so the loop has no divergent exits.  A lane that cannot show that all its
     // samples of the slab have their taps in the box (not seen in regular cone-beam geometry;
     // every lane when the slab is not staged) reads them from global memory.
+    // the step in a VGPR: the march's t update is then one select and one add (not in the
+    // adjoint, which has no register to spare for it)
+    float step_v = P.step;
+    if (MODE != 2) asm volatile("" : "+v"(step_v));
     auto march = [&](auto pax_c, auto fwd_c, int s_lo, int s_hi, const int* lo, int B0, int B1, int B2,
                      bool staged) {
         constexpr int PAX = decltype(pax_c)::value;
@@ -400,7 +404,8 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 const Taps tg = global_taps(x, y, z);
                 use_taps(tg, fx, fy, fz, true);
             }
-            t = go ? t + P.step : t;
+            if (MODE != 2) t += go ? step_v : 0.f;
+            else t = go ? t + P.step : t;
         }
         alive = t <= tmax;
     };

@@ -52,8 +52,8 @@ DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 # Physical HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs of
 # `bench.py --pmc-probe` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB copy in the
 # same runs): adjoint fp_slab_kernel<2> at 256^3 x 720 views, 768x640 detector.
-MEASURED_TRAFFIC = {("geometry-gradients-256c-720v", "adjoint"): 128_418_581 * 1024 * 2.0 + 142_782_761 * 1024,
-                    ("geometry-gradients-256c-720v", "forward"): 113_586_044 * 1024 * 2.0 + 1_463_351 * 1024,
+MEASURED_TRAFFIC = {("geometry-gradients-256c-720v", "adjoint"): 128_434_406 * 1024 * 2.0 + 142_782_328 * 1024,
+                    ("geometry-gradients-256c-720v", "forward"): 113_562_408 * 1024 * 2.0 + 1_465_354 * 1024,
                     # forward at 360 views, 512x384 (FDK round trip); backprojection backward at
                     # 256^3 x 360 views (same counters, same factors)
                     ("fdk-roundtrip-256c-360v", "forward"): 58_575_790 * 1024 * 2.0 + 295_300 * 1024,

@@ -306,6 +306,11 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         return front && lo[0] < hi[0] && lo[1] < hi[1] && lo[2] < hi[2];
     };
 
+    // the step in a VGPR: the march's t update is then one select and one add (not in the
+    // adjoint, which has no register to spare for it)
+    float step_v = P.step;
+    if (MODE != 2) asm volatile("" : "+v"(step_v));
+
     // The samples of one slab: those whose principal floor coordinate lies in [s_lo, s_hi], with
     // the taps read from the staged box.  PAX / FWD (the tile's principal axis and direction) are
     // compile-time constants, so the slab-exit test is one compare.  The loop runs while any lane
@@ -313,10 +318,6 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     // (go == false), so the loop has no divergent exits.  A lane that cannot show that all its
     // samples of the slab have their taps in the box (not seen in regular cone-beam geometry;
     // every lane when the slab is not staged) reads them from global memory.
-    // the step in a VGPR: the march's t update is then one select and one add (not in the
-    // adjoint, which has no register to spare for it)
-    float step_v = P.step;
-    if (MODE != 2) asm volatile("" : "+v"(step_v));
     auto march = [&](auto pax_c, auto fwd_c, int s_lo, int s_hi, const int* lo, int B0, int B1, int B2,
                      bool staged) {
         constexpr int PAX = decltype(pax_c)::value;

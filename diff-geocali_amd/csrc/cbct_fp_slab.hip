@@ -260,7 +260,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
 
     // Box of the tile's samples whose principal floor coordinate lies in [s_lo, s_hi], in raw
     // (unclamped) cell coordinates: the hull of the corner rays between the planes q = s_lo and
-    // q = s_hi + 1, widened by the +1 taps and a rounding margin, limited to [-1, N].  The staged
+    // q = s_hi + 1, widened by the +1 taps and a small rounding margin, limited to [-1, N].  The staged
     // copy holds vol[clamp(.)] at every raw coordinate, so taps read from it need no clamping.
     // Returns false when no cell fits in it.
     // Lateral slopes of the four corner rays (voxel units per principal voxel), once per tile:
@@ -287,10 +287,12 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             mn2 = fminf(mn2, fminf(a2, b2)); mx2 = fmaxf(mx2, fmaxf(a2, b2));
         }
         const int lp = max(-1, s_lo), hp = min(Np, s_hi + 1);
-        const int l1 = (int)fminf(fmaxf(floorf(mn1) - 1.f, -1.f), (float)Nx1);
-        const int h1 = (int)fmaxf(fminf(floorf(mx1) + 2.f, (float)Nx1), -1.f);
-        const int l2 = (int)fminf(fmaxf(floorf(mn2) - 1.f, -1.f), (float)Nx2);
-        const int h2 = (int)fmaxf(fminf(floorf(mx2) + 2.f, (float)Nx2), -1.f);
+        // rounding margin: 0.01 voxel, far above the hull's FP32 error (~1e-4 voxel); a sample
+        // that still fell outside would only send its lane to the global path (in-box proof)
+        const int l1 = (int)fminf(fmaxf(floorf(mn1 - 0.01f), -1.f), (float)Nx1);
+        const int h1 = (int)fmaxf(fminf(floorf(mx1 + 0.01f) + 1.f, (float)Nx1), -1.f);
+        const int l2 = (int)fminf(fmaxf(floorf(mn2 - 0.01f), -1.f), (float)Nx2);
+        const int h2 = (int)fmaxf(fminf(floorf(mx2 + 0.01f) + 1.f, (float)Nx2), -1.f);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             lo[k] = k == p ? lp : (k == o1 ? l1 : l2);

@@ -52,11 +52,11 @@ DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 # Physical HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs of
 # `bench.py --pmc-probe` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB copy in the
 # same runs): adjoint fp_slab_kernel<2> at 256^3 x 720 views, 768x640 detector.
-MEASURED_TRAFFIC = {("geometry-gradients-256c-720v", "adjoint"): 128_434_406 * 1024 * 2.0 + 142_782_328 * 1024,
-                    ("geometry-gradients-256c-720v", "forward"): 113_562_408 * 1024 * 2.0 + 1_465_354 * 1024,
+MEASURED_TRAFFIC = {("geometry-gradients-256c-720v", "adjoint"): 103_239_016 * 1024 * 2.0 + 137_697_478 * 1024,
+                    ("geometry-gradients-256c-720v", "forward"): 91_308_190 * 1024 * 2.0 + 1_465_094 * 1024,
                     # forward at 360 views, 512x384 (FDK round trip); backprojection backward at
                     # 256^3 x 360 views (same counters, same factors)
-                    ("fdk-roundtrip-256c-360v", "forward"): 58_575_790 * 1024 * 2.0 + 295_300 * 1024,
+                    ("fdk-roundtrip-256c-360v", "forward"): 47_640_337 * 1024 * 2.0 + 296_104 * 1024,
                     ("backprojection-256c-360v", "bp_backward"): 1_595_804 * 1024 * 2.0 + 6_993_152 * 1024,
                     ("backprojection-256c-360v", "bp_forward"): 8_328_817 * 1024 * 2.0 + 172_207 * 1024}
 PHANTOM_SEED = 20261203

@@ -44,7 +44,10 @@ __global__ void calib_frames_kernel(const CalibDev* __restrict__ probs, int npro
     const int a = gv - d.view_offset, A = d.A;
     const float* g = d.geom;
     const GeomPtrs gp{g, g + A, g + 2 * A, g + 3 * A, g + 4 * A, g + 5 * A, g + 6 * A, g + 7 * A};
-    const_cast<ViewFrame*>(d.s.frames)[a] = make_frame(gp, a);
+    // the geometry being fitted can leave the range that a march can follow (or become NaN, e.g.
+    // from a NaN measurement): such a view is degenerate for this step (zero projection and
+    // gradient) instead of an endless march
+    const_cast<ViewFrame*>(d.s.frames)[a] = guard_frame(gp, a, d.guard);
 }
 
 // Per view: fixed-order sum of the tile partials.  NP = 9 (Adam: 8 geometry gradients + loss,
@@ -281,6 +284,12 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
         if ((st = check_positive_finite(q.pixel_size, "pixel_size"))) return st;
         if ((st = check_positive_finite(q.step_size, "step_size"))) return st;
         if ((st = check_positive_finite(q.voxel_size, "voxel_size"))) return st;
+        if ((st = check_finite(q.off_x, "off_x")) || (st = check_finite(q.off_y, "off_y")) ||
+            (st = check_finite(q.off_z, "off_z")))
+            return st;
+        const FrameGuard guard = make_guard_host(q.D, q.H, q.W, q.voxel_size, q.off_x, q.off_y, q.off_z, q.U, q.V,
+                                                 q.pixel_size, q.step_size);
+        if ((st = check_step_resolution(q.step_size, guard))) return st;
         CalibDev& d = host[i];
         SlabArgs& s = d.s;
         s.vol = q.volume;
@@ -308,6 +317,7 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
         d.solver = solver;
         d.ne = solver == 1 ? (float*)(ws + L.ne[i]) : nullptr;
         d.lm = solver == 1 ? (float*)(ws + L.lm[i]) : nullptr;
+        d.guard = guard;
         d.view_offset = total_views;
         d.tile_offset = total_tiles;
         d.param_offset = total_params;

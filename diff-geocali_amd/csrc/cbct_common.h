@@ -48,7 +48,7 @@ struct alignas(16) ViewFrame {
     float sod, sid;
     float dsu, dsn, dsv; // source offset in the rotating frame (backprojector)
     float n;             // sod - sid
-    float pad;
+    float bad;           // 1: degenerate view (see guard_frame), every other field 0
 };
 static_assert(sizeof(ViewFrame) == 64, "ViewFrame is 16 floats");
 
@@ -77,7 +77,76 @@ __device__ __forceinline__ ViewFrame make_frame(const GeomPtrs& g, int a) {
     f.dsu = fmaf(sx, f.c, sy * f.s);
     f.dsn = fmaf(-sx, f.s, sy * f.c);
     f.dsv = sz;
-    f.pad = 0.f;
+    f.bad = 0.f;
+    return f;
+}
+
+// Which views the projector can march.  A view is degenerate -- zero projections and zero
+// gradients, as the reference does for a degenerate ray
+// (csrc/src_cbct_forwardprojector_cuda.cu:159-161) -- unless
+//   (1) its eight geometry values and the frame made from them are finite;
+//   (2) the unnormalised directions of its detector's corner rays have every component below
+//       1e18 in magnitude.  rx and ry depend on u only, rz on v only, each through correctly
+//       rounded operations with fixed-sign coefficients, so each is monotone in the pixel
+//       index and every ray of the detector is bounded by the corners: its squared length is
+//       finite and so is its unit direction;
+//   (3) step >= 2^-22 * Tb with Tb = 2 (|O - C| + R) + 1, where O is the source, C the volume
+//       box's centre and R its half-diagonal.  Every t of a sample (0 <= t <= t_max, and t_max
+//       of a ray that meets the box is below Tb even with the slab test's rounding) is then
+//       below Tb, so fl(t + step) >= t + step/2: each march ends after at most
+//       2 (t_max - t_min) / step + 1 <= 2^23 + 1 samples.  (The reference's loop never ends when
+//       step is below the spacing of floats at t, or when a NaN direction gives [0, FLT_MAX].)
+// A degenerate view's frame is all zeros with bad = 1, so no later arithmetic meets a NaN or
+// an infinity.  The tests must see NaNs: call guard_frame only from code compiled with NaN
+// semantics (not under -fno-honor-nans).
+struct FrameGuard {
+    int on;                // 0: plain frames (the backprojector, whose loops are bounded by A)
+    int U, V;
+    float pix, step;
+    float cx, cy, cz, R;   // volume box centre and half-diagonal, world units
+};
+
+inline FrameGuard make_guard_host(int64_t D, int64_t H, int64_t W, float vs, float ox, float oy, float oz,
+                                  int64_t U, int64_t V, float pix, float step) {
+    FrameGuard G;
+    G.on = 1;
+    G.U = (int)U; G.V = (int)V;
+    G.pix = pix; G.step = step;
+    // box [(-N/2 - 1/2) vs + off, (N/2 - 1/2) vs + off] per axis
+    G.cx = (float)((double)ox - 0.5 * (double)vs);
+    G.cy = (float)((double)oy - 0.5 * (double)vs);
+    G.cz = (float)((double)oz - 0.5 * (double)vs);
+    G.R = (float)(0.5 * (double)vs * std::sqrt((double)W * W + (double)H * H + (double)D * D));
+    return G;
+}
+
+__device__ inline ViewFrame guard_frame(const GeomPtrs& g, int a, const FrameGuard& G) {
+    ViewFrame f = make_frame(g, a);
+    if (!G.on) return f;
+    bool ok = isfinite(g.ang[a]) && isfinite(g.sid[a]) && isfinite(g.sod[a]) && isfinite(g.u0[a]) &&
+              isfinite(g.v0[a]) && isfinite(g.sx[a]) && isfinite(g.sy[a]) && isfinite(g.sz[a]);
+    ok = ok && isfinite(f.c) && isfinite(f.s) && isfinite(f.ox) && isfinite(f.oy) && isfinite(f.oz) &&
+         isfinite(f.ns) && isfinite(f.nc) && isfinite(f.n) && isfinite(f.dsu) && isfinite(f.dsn);
+    const float lim = 1e18f;
+    for (int i = 0; i < 2; ++i) {
+        // the expressions of make_ray at u = 0, U-1 and v = 0, V-1
+        const float up = ((i ? (float)(G.U - 1) : 0.f) - f.u0) * G.pix;
+        const float vp = ((i ? (float)(G.V - 1) : 0.f) - f.v0) * G.pix;
+        const float rx = fmaf(up, f.c, -f.ns) - f.ox;
+        const float ry = fmaf(up, f.s, f.nc) - f.oy;
+        const float rz = vp - f.oz;
+        ok = ok && fabsf(rx) <= lim && fabsf(ry) <= lim && fabsf(rz) <= lim;
+    }
+    if (ok) {
+        const double dx = (double)f.ox - (double)G.cx, dy = (double)f.oy - (double)G.cy;
+        const double dz = (double)f.oz - (double)G.cz;
+        const double Tb = 2.0 * (sqrt(dx * dx + dy * dy + dz * dz) + (double)G.R) + 1.0;
+        ok = (double)G.step >= Tb * 0x1p-22;
+    }
+    if (!ok) {
+        f = ViewFrame{};
+        f.bad = 1.f;
+    }
     return f;
 }
 
@@ -111,6 +180,7 @@ struct Ray {
 // Unit direction from the source through detector pixel (u,v); false for a
 // degenerate ray (csrc/src_cbct_forwardprojector_cuda.cu:138-169).
 __device__ __forceinline__ bool make_ray(const ViewFrame& f, int u, int v, float pix, Ray& r) {
+    if (f.bad != 0.f) return false;        // degenerate view (guard_frame)
     const float up = ((float)u - f.u0) * pix;
     const float vp = ((float)v - f.v0) * pix;
     const float tx = fmaf(up, f.c, -f.ns);
@@ -286,10 +356,15 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 int check_geometry(const float* const* g, int64_t A);
 int check_positive_finite(float x, const char* name);
 int check_finite(float x, const char* name);
+// step_size against the FP32 resolution of the ray parameter over the volume: a step below
+// 2^-22 (volume diagonal + 1) leaves every view degenerate (guard_frame), so it is refused.
+int check_step_resolution(float step, const FrameGuard& G);
 
-__global__ void frames_kernel(GeomPtrs g, int A, ViewFrame* __restrict__ out);
+__global__ void frames_kernel(GeomPtrs g, int A, FrameGuard guard, ViewFrame* __restrict__ out);
 __global__ void fill_kernel(float* __restrict__ p, int64_t n, float value);
-int launch_frames(const GeomPtrs& g, int64_t A, ViewFrame* out, hipStream_t stream);
+// Frames of the A views; with a guard (projector) degenerate views are flagged (guard_frame).
+int launch_frames(const GeomPtrs& g, int64_t A, ViewFrame* out, hipStream_t stream,
+                  const FrameGuard* guard = nullptr);
 int launch_fill(float* p, int64_t n, float value, hipStream_t stream);
 // Kernels of cbct_fp.hip shared with the backprojector backward: zero n int64 words; max |x|
 // over n floats into *maxbits (as float bits, *maxbits cleared beforehand); ggeom[k][a] = the

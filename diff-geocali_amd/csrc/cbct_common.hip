@@ -40,9 +40,16 @@ int check_finite(float x, const char* name) {
     return CBCT_OK;
 }
 
-__global__ void frames_kernel(GeomPtrs g, int A, ViewFrame* __restrict__ out) {
+int check_step_resolution(float step, const FrameGuard& G) {
+    if ((double)step < (2.0 * (double)G.R + 1.0) * 0x1p-22)
+        return fail(CBCT_EINVAL, "step_size is below the FP32 resolution of the ray parameter over the volume "
+                                 "(it must be at least 2^-22 * (volume diagonal + 1))");
+    return CBCT_OK;
+}
+
+__global__ void frames_kernel(GeomPtrs g, int A, FrameGuard guard, ViewFrame* __restrict__ out) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < A) out[a] = make_frame(g, a);
+    if (a < A) out[a] = guard_frame(g, a, guard);
 }
 
 __global__ void fill_kernel(float* __restrict__ p, int64_t n, float value) {
@@ -50,8 +57,9 @@ __global__ void fill_kernel(float* __restrict__ p, int64_t n, float value) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = value;
 }
 
-int launch_frames(const GeomPtrs& g, int64_t A, ViewFrame* out, hipStream_t stream) {
-    frames_kernel<<<(unsigned)ceil_div(A, 64), 64, 0, stream>>>(g, (int)A, out);
+int launch_frames(const GeomPtrs& g, int64_t A, ViewFrame* out, hipStream_t stream, const FrameGuard* guard) {
+    FrameGuard none{};
+    frames_kernel<<<(unsigned)ceil_div(A, 64), 64, 0, stream>>>(g, (int)A, guard ? *guard : none, out);
     CBCT_LAUNCHED("view-frame kernel launch");
     return CBCT_OK;
 }

@@ -312,7 +312,7 @@ static int check_fp_common(int64_t D, int64_t H, int64_t W, int64_t A, int64_t U
     if ((st = check_finite(ox, "off_x"))) return st;
     if ((st = check_finite(oy, "off_y"))) return st;
     if ((st = check_finite(oz, "off_z"))) return st;
-    return CBCT_OK;
+    return check_step_resolution(step, make_guard_host(D, H, W, vs, ox, oy, oz, U, V, pix, step));
 }
 
 }  // namespace cbct
@@ -342,7 +342,8 @@ extern "C" int cbct_fp_forward(const float* volume, int64_t D, int64_t H, int64_
     hipStream_t stream = (hipStream_t)stream_;
     ViewFrame* frames = (ViewFrame*)workspace;
     const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
-    if ((st = launch_frames(gp, A, frames, stream))) return st;
+    const FrameGuard guard = make_guard_host(D, H, W, voxel_size, off_x, off_y, off_z, U, V, pixel_size, step_size);
+    if ((st = launch_frames(gp, A, frames, stream, &guard))) return st;
     const Box b = make_box_host(D, H, W, voxel_size, off_x, off_y, off_z);
     const int tu = (int)ceil_div(U, FP_TILE), tv = (int)ceil_div(V, FP_TILE);
     const int nblocks = (int)(A * tu * tv);
@@ -413,7 +414,8 @@ extern "C" int cbct_fp_backward(const float* grad_projection, const float* volum
     ViewFrame* frames = (ViewFrame*)workspace;
     float* partials = (float*)((char*)workspace + align256((size_t)A * sizeof(ViewFrame)));
     const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
-    if ((st = launch_frames(gp, A, frames, stream))) return st;
+    const FrameGuard guard = make_guard_host(D, H, W, voxel_size, off_x, off_y, off_z, U, V, pixel_size, step_size);
+    if ((st = launch_frames(gp, A, frames, stream, &guard))) return st;
     unsigned long long* gacc = nullptr;
     int* gexp = nullptr;
     const int64_t nvox = D * H * W;
@@ -480,7 +482,8 @@ extern "C" int cbct_fp_count_samples(int64_t D, int64_t H, int64_t W,
     hipStream_t stream = (hipStream_t)stream_;
     ViewFrame* frames = (ViewFrame*)workspace;
     const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
-    if ((st = launch_frames(gp, A, frames, stream))) return st;
+    const FrameGuard guard = make_guard_host(D, H, W, voxel_size, off_x, off_y, off_z, U, V, pixel_size, step_size);
+    if ((st = launch_frames(gp, A, frames, stream, &guard))) return st;
     const Box b = make_box_host(D, H, W, voxel_size, off_x, off_y, off_z);
     const int tu = (int)ceil_div(U, FP_TILE), tv = (int)ceil_div(V, FP_TILE);
     const int nblocks = (int)(A * tu * tv);

@@ -43,6 +43,7 @@ struct CalibDev {
     int solver;          // 0: Adam on the gradient; 1: Levenberg-Marquardt on the normal equations
     float* ne;           // [A][45] normal equations of the current geometry (solver 1)
     float* lm;           // [A][LM_STATE] accepted geometry, its normal equations, damping, loss
+    FrameGuard guard;    // which views can be marched (guard_frame)
 };
 
 constexpr int LM_NE = 45;               // 36 (upper triangle of J^T J) + 8 (J^T r) + loss

@@ -111,6 +111,16 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def check_step_resolution(volume_shape, step_size, voxel_size):
+    """The library's host check: a step below 2^-22 (volume diagonal + 1) is refused (every view
+    would be degenerate, since no march could follow it in FP32)."""
+    D, H, W = volume_shape
+    diag = float(voxel_size) * math.sqrt(float(W) ** 2 + float(H) ** 2 + float(D) ** 2)
+    R = float(np.float32(0.5 * diag))
+    if float(step_size) < (2.0 * R + 1.0) * 2.0 ** -22:
+        raise ValueError("step_size is below the FP32 resolution of the ray parameter over the volume")
+
+
 def fp_forward(volume, geometry, pixel_size, U, V, step_size, voxel_size,
                offset=(0.0, 0.0, 0.0), mode="f32", traces=False, threads=None):
     """Ray integrals [A,U,V]; with ``traces`` also (entry_axis, count, cell_hash) per ray."""
@@ -119,6 +129,7 @@ def fp_forward(volume, geometry, pixel_size, U, V, step_size, voxel_size,
     g = _geom(geometry, dt)
     A = g[0].size
     D, H, W = vol.shape
+    check_step_resolution((D, H, W), step_size, voxel_size)
     out = np.zeros((A, U, V), dt)
     tr = (np.zeros((A, U, V), np.int32), np.zeros((A, U, V), np.int64), np.zeros((A, U, V), np.uint64)) if traces else None
     fn = getattr(_lib(), f"orc_fp_forward_{mode}")
@@ -135,6 +146,7 @@ def fp_count_samples(volume_shape, geometry, pixel_size, U, V, step_size, voxel_
     dt = _dt(mode)
     g = _geom(geometry, dt)
     D, H, W = volume_shape
+    check_step_resolution((D, H, W), step_size, voxel_size)
     hits = ctypes.c_int64(0)
     fn = getattr(_lib(), f"orc_fp_count_samples_{mode}")
     S = fn(D, H, W, *[_ptr(x) for x in g], g[0].size, pixel_size, U, V, step_size, voxel_size,
@@ -152,6 +164,7 @@ def fp_backward(grad, volume, geometry, pixel_size, step_size, voxel_size,
     g = _geom(geometry, dt)
     A, U, V = gr.shape
     D, H, W = vol.shape
+    check_step_resolution((D, H, W), step_size, voxel_size)
     gvol = np.zeros_like(vol) if need_volume else None
     ggeo = np.zeros((8, A), dt)
     fn = getattr(_lib(), f"orc_fp_backward_{mode}")

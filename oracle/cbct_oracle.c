@@ -23,6 +23,43 @@
 #include <omp.h>
 #endif
 
+/* Which views the projector can march (the library's guard_frame in
+ * diff-geocali_amd/csrc/cbct_common.h, restated with the same float expressions).
+ * A view is degenerate -- zero projections and zero gradients -- unless its eight
+ * geometry values and the frame made from them are finite, the unnormalised
+ * directions of its detector's corner rays stay below 1e18 in every component, and
+ * step >= 2^-22 * (2 (|O - C| + R) + 1) (O the source, C the volume box's centre, R
+ * its half-diagonal), which makes every march end: t then grows by at least step/2
+ * per sample.  Evaluated on the FP32 values in both instantiations, so the FP32
+ * schedule and the FP64 oracle flag the same views. */
+static int guard_view_f32(float ang, float sid, float sod, float u0, float v0, float sx, float sy, float sz,
+                          int D, int H, int W, float vs, float offx, float offy, float offz,
+                          int U, int V, float pix, float step) {
+    if (!(isfinite(ang) && isfinite(sid) && isfinite(sod) && isfinite(u0) && isfinite(v0) &&
+          isfinite(sx) && isfinite(sy) && isfinite(sz)))
+        return 0;
+    const float c = (float)cos((double)ang), s = (float)sin((double)ang);
+    const float n = sod - sid;
+    const float ox = fmaf(-sod, s, sx), oy = fmaf(sod, c, sy), oz = sz;
+    const float ns = n * s, nc = n * c;
+    const float dsu = fmaf(sx, c, sy * s), dsn = fmaf(-sx, s, sy * c);
+    if (!(isfinite(c) && isfinite(s) && isfinite(ox) && isfinite(oy) && isfinite(oz) && isfinite(ns) &&
+          isfinite(nc) && isfinite(n) && isfinite(dsu) && isfinite(dsn)))
+        return 0;
+    for (int i = 0; i < 2; ++i) {
+        const float up = ((i ? (float)(U - 1) : 0.f) - u0) * pix;
+        const float vp = ((i ? (float)(V - 1) : 0.f) - v0) * pix;
+        const float rx = fmaf(up, c, -ns) - ox, ry = fmaf(up, s, nc) - oy, rz = vp - oz;
+        if (!(fabsf(rx) <= 1e18f && fabsf(ry) <= 1e18f && fabsf(rz) <= 1e18f)) return 0;
+    }
+    const float cx = (float)((double)offx - 0.5 * (double)vs), cy = (float)((double)offy - 0.5 * (double)vs);
+    const float cz = (float)((double)offz - 0.5 * (double)vs);
+    const float R = (float)(0.5 * (double)vs * sqrt((double)W * W + (double)H * H + (double)D * D));
+    const double dx = (double)ox - cx, dy = (double)oy - cy, dz = (double)oz - cz;
+    const double Tb = 2.0 * (sqrt(dx * dx + dy * dy + dz * dz) + (double)R) + 1.0;
+    return (double)step >= Tb * 0x1p-22;
+}
+
 #define REAL float
 #define SFX(name) name##_f32
 #define RFMA(a, b, c) fmaf((a), (b), (c))

@@ -10,6 +10,7 @@ library is missing every operator raises.
 
 import ctypes
 import os
+import threading
 
 import torch
 
@@ -45,6 +46,15 @@ _SIGNATURES = {
     "cbct_fp_get_tuning": ([_P], _INT),
     "cbct_fp_set_tuning_op": ([_INT, _INT, _INT, _INT, _INT], _INT),
     "cbct_fp_get_tuning_op": ([_INT, _P], _INT),
+    "cbct_fp_default_options": ([_INT, _P], _INT),
+    "cbct_fp_forward_ex": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_I64, _F, _I64, _I64, _F, _F, _F, _F, _F,
+                           _P, _P, _P, _SZ, _P], _INT),
+    "cbct_fp_backward_workspace_bytes_ex": ([_I64] * 6 + [_P], _SZ),
+    "cbct_fp_backward_ex": ([_P, _P, _I64, _I64, _I64] + [_P] * 8 + [_I64, _I64, _I64, _F, _F, _F, _INT,
+                            _F, _F, _F, _P, _P, _P, _P, _SZ, _P], _INT),
+    "cbct_bp_backward_workspace_bytes_ex": ([_I64] * 6 + [_P], _SZ),
+    "cbct_bp_backward_ex": ([_P, _P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
+                            _P, _P, _P, _P, _SZ, _P], _INT),
     "cbct_bp_workspace_bytes": ([_I64, _I64, _I64, _I64], _SZ),
     "cbct_bp_backward_workspace_bytes": ([_I64] * 6, _SZ),
     "cbct_bp_forward": ([_P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
@@ -133,9 +143,101 @@ def _require_gpu(t, what):
 TUNING_GATHER, TUNING_SCATTER = 0, 1
 
 
+class Options(ctypes.Structure):
+    """cbct_fp_options: the kernel selection and deterministic mode of one call."""
+    _fields_ = [("use_slab", _INT), ("slab_thickness", _INT), ("lds_floats", _INT), ("layout", _INT),
+                ("deterministic", _INT)]
+
+    def as_tuple(self):
+        return (self.use_slab, self.slab_thickness, self.lds_floats, self.layout, self.deterministic)
+
+
+def _validate_tuning(use_slab, slab_thickness, lds_floats, layout):
+    """The library's ranges (cbct_fp_options), checked before any setting is kept."""
+    if use_slab not in (0, 1):
+        raise RuntimeError("use_slab must be 0 or 1")
+    if not 1 <= slab_thickness <= 64:
+        raise RuntimeError("slab_thickness must be in [1, 64]")
+    if not 256 <= lds_floats <= 8192:
+        raise RuntimeError("lds_floats must be in [256, 8192]")
+    if layout not in (0, 1):
+        raise RuntimeError("layout must be 0 or 1")
+
+
+# Per-thread settings.  The library's own process-wide defaults are read once per thread;
+# set_fp_tuning / set_deterministic (and their context managers) then change only the calling
+# thread's settings, and every call passes them to the library explicitly (cbct_fp_options), so
+# threads with different settings never see each other's.  The autograd Functions capture the
+# settings when their forward runs, so a backward run by autograd's own threads uses them too.
+_THREAD = threading.local()
+
+
+def _state():
+    st = getattr(_THREAD, "state", None)
+    if st is None:
+        L = lib()
+        slots = []
+        for op in (TUNING_GATHER, TUNING_SCATTER):
+            o = Options()
+            _check(L.cbct_fp_default_options(op, ctypes.byref(o)))
+            slots.append(o.as_tuple()[:4])
+        st = {"tuning": slots, "deterministic": bool(o.deterministic)}
+        _THREAD.state = st
+    return st
+
+
 def set_fp_tuning(use_slab, slab_thickness, lds_floats, layout, op=None):
-    """Select the projector kernels for subsequent calls (same results for every setting), for
-    one tuning slot (``op`` 0 or 1) or both (``op=None``)."""
+    """Select the projector kernels for this thread's later calls (same results for every setting),
+    for one tuning slot (``op`` 0 or 1) or both (``op=None``)."""
+    args = (int(use_slab), int(slab_thickness), int(lds_floats), int(layout))
+    _validate_tuning(*args)
+    st = _state()
+    for k in ((TUNING_GATHER, TUNING_SCATTER) if op is None else (int(op),)):
+        if k not in (TUNING_GATHER, TUNING_SCATTER):
+            raise RuntimeError("op must be 0 (gathering kernels) or 1 (volume-gradient adjoint)")
+        st["tuning"][k] = args
+
+
+def get_fp_tuning(op=TUNING_GATHER):
+    return tuple(_state()["tuning"][int(op)])
+
+
+def set_deterministic(on):
+    """Bitwise-reproducible projector volume gradients and backprojection projection gradients
+    (int64 fixed-point accumulation) for this thread's later calls."""
+    _state()["deterministic"] = bool(on)
+
+
+def get_deterministic():
+    return bool(_state()["deterministic"])
+
+
+def options(op=TUNING_GATHER, tuning=None, deterministic=None):
+    """cbct_fp_options of one call in slot ``op``: this thread's settings, or the given
+    ``tuning`` (use_slab, slab_thickness, lds_floats, layout) / ``deterministic``."""
+    st = _state()
+    t = tuple(int(x) for x in (tuning if tuning is not None else st["tuning"][int(op)]))
+    _validate_tuning(*t)
+    det = st["deterministic"] if deterministic is None else bool(deterministic)
+    return Options(*t, 1 if det else 0)
+
+
+class CallSettings:
+    """A snapshot of this thread's settings (both slots and the deterministic mode), taken by an
+    autograd Function's forward and used again by its backward."""
+
+    def __init__(self):
+        st = _state()
+        self.tuning = [tuple(x) for x in st["tuning"]]
+        self.deterministic = st["deterministic"]
+
+    def options(self, op):
+        return options(op, tuning=self.tuning[op], deterministic=self.deterministic)
+
+
+def set_default_fp_tuning(use_slab, slab_thickness, lds_floats, layout, op=None):
+    """Change the library's process-wide default (read by threads that have not read it yet and by
+    C callers that pass no options)."""
     args = (int(use_slab), int(slab_thickness), int(lds_floats), int(layout))
     if op is None:
         _check(lib().cbct_fp_set_tuning(*args))
@@ -143,23 +245,8 @@ def set_fp_tuning(use_slab, slab_thickness, lds_floats, layout, op=None):
         _check(lib().cbct_fp_set_tuning_op(int(op), *args))
 
 
-def get_fp_tuning(op=TUNING_GATHER):
-    out = (ctypes.c_int * 4)()
-    _check(lib().cbct_fp_get_tuning_op(int(op), ctypes.cast(out, ctypes.c_void_p)))
-    return tuple(out)
-
-
-def set_deterministic(on):
-    """Bitwise-reproducible projector volume gradients (int64 fixed-point accumulation)."""
-    _check(lib().cbct_fp_set_deterministic(1 if on else 0))
-
-
-def get_deterministic():
-    return bool(lib().cbct_fp_get_deterministic())
-
-
 class deterministic:
-    """Context manager for the deterministic volume-gradient mode."""
+    """Context manager for this thread's deterministic mode."""
 
     def __init__(self, on=True):
         self.new = bool(on)
@@ -175,8 +262,8 @@ class deterministic:
 
 
 class fp_tuning:
-    """Context manager that applies a projector tuning (to one slot, or both with ``op=None``)
-    and restores both slots afterwards."""
+    """Context manager that applies a projector tuning to this thread (one slot, or both with
+    ``op=None``) and restores both slots afterwards."""
 
     def __init__(self, use_slab, slab_thickness=8, lds_floats=4096, layout=1, op=None):
         self.new = (use_slab, slab_thickness, lds_floats, layout)
@@ -193,42 +280,47 @@ class fp_tuning:
         return False
 
 
-def fp_forward(volume, geometry, pixel_size, U, V, step_size, voxel_size, off_x, off_y, off_z):
-    """proj[A,U,V]; volume [D,H,W] and the 8 geometry tensors must be contiguous float32."""
+def fp_forward(volume, geometry, pixel_size, U, V, step_size, voxel_size, off_x, off_y, off_z, opts=None):
+    """proj[A,U,V]; volume [D,H,W] and the 8 geometry tensors must be contiguous float32.
+    ``opts``: cbct_fp_options of the call (default: this thread's gather settings)."""
     _require_gpu(volume, "volume")
     L = lib()
     D, H, W = volume.shape
     A = geometry[0].numel()
     dev = volume.device
+    o = opts if opts is not None else options(TUNING_GATHER)
     with torch.cuda.device(dev):
         out = torch.empty((A, U, V), dtype=torch.float32, device=dev)
         ws_bytes = L.cbct_fp_workspace_bytes(A, U, V)
         ws = _workspace(ws_bytes, dev)
-        _check(L.cbct_fp_forward(_ptr(volume), D, H, W, *[_ptr(g) for g in geometry], A,
-                                 pixel_size, U, V, step_size, voxel_size, off_x, off_y, off_z,
-                                 _ptr(out), _ptr(ws), ws_bytes, _stream(dev)))
+        _check(L.cbct_fp_forward_ex(_ptr(volume), D, H, W, *[_ptr(g) for g in geometry], A,
+                                    pixel_size, U, V, step_size, voxel_size, off_x, off_y, off_z,
+                                    _ptr(out), ctypes.byref(o), _ptr(ws), ws_bytes, _stream(dev)))
     return out
 
 
 def fp_backward(grad, volume, geometry, pixel_size, step_size, voxel_size, compute_grad_volume,
-                off_x, off_y, off_z):
-    """(gvol [D,H,W] or empty, 8 geometry gradients [A]) of sum(grad * fp_forward)."""
+                off_x, off_y, off_z, compute_grad_geometry=True, opts=None):
+    """(gvol [D,H,W] or empty, [8, A] geometry gradients or None) of sum(grad * fp_forward).
+    ``opts``: cbct_fp_options of the call (default: this thread's settings of the slot the call
+    runs in)."""
     _require_gpu(volume, "volume")
     L = lib()
     D, H, W = volume.shape
     A, U, V = grad.shape
     dev = volume.device
+    o = opts if opts is not None else options(TUNING_SCATTER if compute_grad_volume else TUNING_GATHER)
     with torch.cuda.device(dev):
         gvol = (torch.empty((D, H, W), dtype=torch.float32, device=dev) if compute_grad_volume
                 else torch.empty((0,), dtype=torch.float32, device=dev))
-        ggeo = torch.empty((8, A), dtype=torch.float32, device=dev)
-        ws_bytes = L.cbct_fp_backward_workspace_bytes(D, H, W, A, U, V)
+        ggeo = torch.empty((8, A), dtype=torch.float32, device=dev) if compute_grad_geometry else None
+        ws_bytes = L.cbct_fp_backward_workspace_bytes_ex(D, H, W, A, U, V, ctypes.byref(o))
         ws = _workspace(ws_bytes, dev)
-        _check(L.cbct_fp_backward(_ptr(grad), _ptr(volume), D, H, W, *[_ptr(g) for g in geometry],
-                                  A, U, V, pixel_size, step_size, voxel_size,
-                                  1 if compute_grad_volume else 0, off_x, off_y, off_z,
-                                  _ptr(gvol) if compute_grad_volume else None, _ptr(ggeo),
-                                  _ptr(ws), ws_bytes, _stream(dev)))
+        _check(L.cbct_fp_backward_ex(_ptr(grad), _ptr(volume), D, H, W, *[_ptr(g) for g in geometry],
+                                     A, U, V, pixel_size, step_size, voxel_size,
+                                     1 if compute_grad_volume else 0, off_x, off_y, off_z,
+                                     _ptr(gvol) if compute_grad_volume else None, _ptr(ggeo), ctypes.byref(o),
+                                     _ptr(ws), ws_bytes, _stream(dev)))
     return gvol, ggeo
 
 
@@ -265,20 +357,23 @@ def bp_forward(proj, geometry, pixel_size, D, H, W, voxel_size, off_x, off_y, of
     return vol
 
 
-def bp_backward(grad_vol, proj, geometry, pixel_size, voxel_size, off_x, off_y, off_z):
+def bp_backward(grad_vol, proj, geometry, pixel_size, voxel_size, off_x, off_y, off_z, opts=None):
+    """(grad_projections [A,U,V], [8, A] geometry gradients); ``opts``: cbct_fp_options of the
+    call (only its deterministic flag matters; default: this thread's setting)."""
     _require_gpu(proj, "projections")
     L = lib()
     A, U, V = proj.shape
     D, H, W = grad_vol.shape
     dev = proj.device
+    o = opts if opts is not None else options(TUNING_GATHER)
     with torch.cuda.device(dev):
         gproj = torch.empty((A, U, V), dtype=torch.float32, device=dev)
         ggeo = torch.empty((8, A), dtype=torch.float32, device=dev)
-        ws_bytes = L.cbct_bp_backward_workspace_bytes(A, U, V, D, H, W)
+        ws_bytes = L.cbct_bp_backward_workspace_bytes_ex(A, U, V, D, H, W, ctypes.byref(o))
         ws = _workspace(ws_bytes, dev)
-        _check(L.cbct_bp_backward(_ptr(grad_vol), _ptr(proj), A, U, V, *[_ptr(g) for g in geometry],
-                                  pixel_size, D, H, W, voxel_size, off_x, off_y, off_z,
-                                  _ptr(gproj), _ptr(ggeo), _ptr(ws), ws_bytes, _stream(dev)))
+        _check(L.cbct_bp_backward_ex(_ptr(grad_vol), _ptr(proj), A, U, V, *[_ptr(g) for g in geometry],
+                                     pixel_size, D, H, W, voxel_size, off_x, off_y, off_z,
+                                     _ptr(gproj), _ptr(ggeo), ctypes.byref(o), _ptr(ws), ws_bytes, _stream(dev)))
     return gproj, ggeo
 
 

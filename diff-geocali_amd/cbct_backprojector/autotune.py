@@ -128,17 +128,19 @@ def tune_projector(volume, geometry, pixel_size, U, V, step_size, voxel_size, op
         gen = torch.Generator(device=volume.device).manual_seed(3)
         up = torch.randn((n, U, V), generator=gen, device=volume.device)
 
-    def run():
+    def run(opts):
         if op == "forward":
-            _native.fp_forward(volume, sub, pixel_size, U, V, step_size, voxel_size, 0., 0., 0.)
+            _native.fp_forward(volume, sub, pixel_size, U, V, step_size, voxel_size, 0., 0., 0., opts=opts)
         else:
             _native.fp_backward(up, volume, sub, pixel_size, step_size, voxel_size, op == "adjoint",
-                                0., 0., 0.)
+                                0., 0., 0., opts=opts)
 
     timings = {}
     for c in (space or candidates()):
-        with _native.fp_tuning(1, *c):
-            timings[c] = _median_ms(run, reps)
+        # each candidate is passed to its calls explicitly: no setting of this or any other
+        # thread changes while the tuner runs
+        opts = _native.options(tuning=(1, *c), deterministic=False)
+        timings[c] = _median_ms(lambda: run(opts), reps)
     best = min(timings, key=timings.get)
     if use_cache and space is None:
         _store(key, {"best": list(best), "ms_per_view": timings[best] / n,
@@ -174,6 +176,11 @@ def tune_calibration(problem, views=8, reps=2, use_cache=True, space=None):
 
 
 def apply(tuning, op=None):
-    """Make ``tuning`` the process-wide projector setting of the slot that ``op`` ("forward",
+    """Make ``tuning`` the calling thread's projector setting of the slot that ``op`` ("forward",
     "geometry", "adjoint", "calibration") runs in, or of both slots (``op=None``)."""
     _native.set_fp_tuning(1, *tuning, op=None if op is None else _SLOT[op])
+
+
+def options_for(tuning, op):
+    """cbct_fp_options that run ``op`` with ``tuning`` (slab kernels), for passing to one call."""
+    return _native.options(_SLOT[op], tuning=(1, *tuning))

@@ -91,7 +91,9 @@ class CalibrationProblem:
         self.grad = torch.zeros_like(self.geometry) if want_grad else None
         D, H, W = self.volume.shape
         A, U, V = self.measured.shape
-        t = self.tuning or (0, 0, 0)
+        # this problem's tuning, or the calling thread's setting for the gathering kernels: the
+        # library is never left to read its process-wide default
+        t = self.tuning or _native.get_fp_tuning(_native.TUNING_GATHER)[1:]
         return _CalibProblem(
             self.volume.data_ptr(), D, H, W, self.voxel_size, *self.volume_offset,
             self.measured.data_ptr(), A, U, V, self.pixel_size, self.step_size,

@@ -1,135 +1,116 @@
-"""First-order autograd wrapper for the source-only cone-beam ray integrator.
+"""Cone-beam ray-integral projector with a first-order autograd node.
 
-Mirrors cbct_backprojector/source_only_forward_projector.py of the reference
-(same names, defaults, validation and error types); the arithmetic runs in
-``libcbct_hip.so`` (``cbct_fp_forward`` / ``cbct_fp_backward``).
+Public names, signatures, defaults and errors are those of the reference package's
+``cbct_backprojector/source_only_forward_projector.py`` (``project`` and
+``DifferentiableSourceOnlyForwardProjector``); the arithmetic is ``cbct_fp_forward_ex`` /
+``cbct_fp_backward_ex`` of ``libcbct_hip.so``.
 """
 
-import math
-from numbers import Integral, Real
+from collections import namedtuple
 
-import torch
 from torch.autograd import Function
+from torch.autograd.function import once_differentiable
 
-from . import _native
+from . import _arguments, _native
 from .operator import _positive_scalar
+
+_TENSOR_NAMES = ("volume", "angles", "sid", "sod", "det_u0_arr", "det_v0_arr",
+                 "src_x_arr", "src_y_arr", "src_z_arr")
+
+# The checked scalar settings of one projection.
+_Setup = namedtuple("_Setup", "pixel U V step voxel offset")
 
 
 def _validate_inputs(volume, angles, sid, sod, det_u0_arr, det_v0_arr,
                      src_x_arr, src_y_arr, src_z_arr, pixel_size,
                      detector_u, detector_v, step_size, voxel_size,
                      off_x, off_y, off_z):
-    """Type, device, shape and scalar checks (source_only_forward_projector.py:30-71)."""
-    names = ("volume", "angles", "sid", "sod", "det_u0_arr", "det_v0_arr",
-             "src_x_arr", "src_y_arr", "src_z_arr")
-    tensors = (volume, angles, sid, sod, det_u0_arr, det_v0_arr,
-               src_x_arr, src_y_arr, src_z_arr)
-    for name, tensor in zip(names, tensors):
-        if not isinstance(tensor, torch.Tensor):
-            raise TypeError(f"{name} must be a torch.Tensor")
-        if tensor.dtype != torch.float32 or tensor.device.type != "cuda":
-            raise ValueError(f"{name} must be a CUDA float32 tensor")
-        if tensor.device != volume.device:
-            raise ValueError("All tensors must be on the same CUDA device")
-    if volume.ndim != 3 or any(size < 1 for size in volume.shape):
+    """Check every argument of ``project`` in the reference's order (tensors, shapes, detector
+    size, ROI offsets, then the three positive scalars) and return
+    (pixel_size, detector_u, detector_v, step_size, voxel_size, off_x, off_y, off_z) as plain
+    numbers (interface of cbct_backprojector/source_only_forward_projector.py:30-71)."""
+    tensors = (volume, angles, sid, sod, det_u0_arr, det_v0_arr, src_x_arr, src_y_arr, src_z_arr)
+    _arguments.gpu_float32(zip(_TENSOR_NAMES, tensors), volume)
+    if volume.ndim != 3 or min(volume.shape) < 1:
         raise ValueError("volume must have a nonempty [D,H,W] shape")
     if angles.ndim != 1 or angles.numel() < 1:
         raise ValueError("angles must have a nonempty shape [A]")
-    num_angles = angles.numel()
-    for name, tensor in zip(names[1:], tensors[1:]):
-        if tensor.shape != (num_angles,):
-            raise ValueError(f"{name} must have shape [{num_angles}]")
-    for name, size in (("detector_u", detector_u), ("detector_v", detector_v)):
-        if isinstance(size, bool) or not isinstance(size, Integral):
-            raise TypeError(f"{name} must be a positive integer")
-        if size < 1:
-            raise ValueError(f"{name} must be a positive integer")
-    offsets = (off_x, off_y, off_z)
-    for name, offset in zip(("off_x", "off_y", "off_z"), offsets):
-        if isinstance(offset, bool) or not isinstance(offset, Real):
-            raise TypeError(f"{name} must be a finite Python number")
-        if not math.isfinite(offset):
-            raise ValueError(f"{name} must be finite")
-    return (
-        _positive_scalar(pixel_size, "pixel_size"),
-        int(detector_u),
-        int(detector_v),
-        _positive_scalar(step_size, "step_size"),
-        _positive_scalar(voxel_size, "voxel_size"),
-        *(float(offset) for offset in offsets),
-    )
+    _arguments.one_value_per_view(zip(_TENSOR_NAMES[1:], tensors[1:]), angles.numel())
+    rows = _arguments.detector_extent("detector_u", detector_u)
+    cols = _arguments.detector_extent("detector_v", detector_v)
+    offsets = tuple(_arguments.finite_number(name, value)
+                    for name, value in (("off_x", off_x), ("off_y", off_y), ("off_z", off_z)))
+    return (_positive_scalar(pixel_size, "pixel_size"), rows, cols,
+            _positive_scalar(step_size, "step_size"), _positive_scalar(voxel_size, "voxel_size"),
+            *offsets)
 
 
 class DifferentiableSourceOnlyForwardProjector(Function):
-    """Autograd Function of the ray integrator (source_only_forward_projector.py:74-116)."""
+    """Autograd node of the projector.  ``apply`` takes the arguments of ``project``
+    positionally; the three ROI offsets may be left out (they default to 0), as in the
+    reference (source_only_forward_projector.py:74-116).  Gradients: the volume (the
+    projector's exact transpose, computed only when the volume needs one) and the eight
+    geometry tensors (computed only when one of them needs one)."""
 
     @staticmethod
     def forward(ctx, volume, angles, sid, sod, det_u0_arr, det_v0_arr,
                 src_x_arr, src_y_arr, src_z_arr, pixel_size,
                 detector_u, detector_v, step_size, voxel_size,
                 off_x=0.0, off_y=0.0, off_z=0.0):
-        (pixel_size, detector_u, detector_v, step_size, voxel_size,
-         off_x, off_y, off_z) = _validate_inputs(
-            volume, angles, sid, sod, det_u0_arr, det_v0_arr,
-            src_x_arr, src_y_arr, src_z_arr, pixel_size,
-            detector_u, detector_v, step_size, voxel_size,
-            off_x, off_y, off_z,
-        )
-        tensors = tuple(
-            tensor.contiguous()
-            for tensor in (
-                volume, angles, sid, sod, det_u0_arr, det_v0_arr,
-                src_x_arr, src_y_arr, src_z_arr,
-            )
-        )
-        ctx.save_for_backward(*tensors)
-        ctx.scalars = (pixel_size, step_size, voxel_size, off_x, off_y, off_z)
-        # Direct class.apply(...) callers may omit the optional ROI arguments.
-        ctx.num_inputs = len(ctx.needs_input_grad)
-        return _native.fp_forward(tensors[0], tensors[1:], pixel_size, detector_u, detector_v,
-                                  step_size, voxel_size, off_x, off_y, off_z)
+        checked = _validate_inputs(volume, angles, sid, sod, det_u0_arr, det_v0_arr,
+                                   src_x_arr, src_y_arr, src_z_arr, pixel_size,
+                                   detector_u, detector_v, step_size, voxel_size, off_x, off_y, off_z)
+        setup = _Setup(*checked[:5], checked[5:])
+        dense = [t.contiguous() for t in (volume, angles, sid, sod, det_u0_arr, det_v0_arr,
+                                          src_x_arr, src_y_arr, src_z_arr)]
+        ctx.save_for_backward(*dense)
+        ctx.setup = setup
+        # the calling thread's kernel settings, reused by the backward (which autograd may run
+        # on another thread)
+        ctx.settings = _native.CallSettings()
+        # how many gradients backward must return: 14 when apply() omitted the offsets, else 17
+        ctx.arity = len(ctx.needs_input_grad)
+        return _native.fp_forward(dense[0], dense[1:], setup.pixel, setup.U, setup.V, setup.step, setup.voxel,
+                                  *setup.offset, opts=ctx.settings.options(_native.TUNING_GATHER))
 
     @staticmethod
-    @torch.autograd.function.once_differentiable
+    @once_differentiable
     def backward(ctx, grad_projection):
-        pixel_size, step_size, voxel_size, off_x, off_y, off_z = ctx.scalars
         volume, *geometry = ctx.saved_tensors
+        setup = ctx.setup
+        wanted = ctx.needs_input_grad
+        need_volume, need_geometry = bool(wanted[0]), any(wanted[1:9])
+        slot = _native.TUNING_SCATTER if need_volume else _native.TUNING_GATHER
         grad_volume, grad_geometry = _native.fp_backward(
-            grad_projection.contiguous(), volume, geometry, pixel_size, step_size, voxel_size,
-            bool(ctx.needs_input_grad[0]), off_x, off_y, off_z,
-        )
-        gradients = (grad_volume, *grad_geometry.unbind(0))
-        result = tuple(
-            gradient if needed else None
-            for gradient, needed in zip(gradients, ctx.needs_input_grad[:9])
-        ) + (None,) * 8
-        return result[:ctx.num_inputs]
+            grad_projection.contiguous(), volume, geometry, setup.pixel, setup.step, setup.voxel,
+            need_volume, *setup.offset, compute_grad_geometry=need_geometry, opts=ctx.settings.options(slot))
+        grads = [grad_volume if need_volume else None]
+        grads += [grad_geometry[k] if wanted[1 + k] else None for k in range(8)]
+        grads += [None] * (ctx.arity - len(grads))
+        return tuple(grads)
 
 
 def project(volume, angles, sid, sod, det_u0_arr, det_v0_arr,
             src_x_arr, src_y_arr, src_z_arr, pixel_size,
             detector_u, detector_v, step_size=0.2, voxel_size=1.0,
             off_x=0.0, off_y=0.0, off_z=0.0):
-    """Return physical ray-integral projections with layout [A,U,V] (source_only_forward_projector.py:119-145).
+    """Line integrals of ``volume`` along the rays of a cone-beam scan, as [A, U, V] projections.
 
-    All nine tensors must be float32 on the same GPU device. ``volume`` has
-    shape [D,H,W], and each of the eight geometry tensors has shape [A].
-    ``sid`` is the total SDD; source offsets use world coordinates and move
-    only the source. The detector stays at local y = SOD - SDD. Angles are
-    radians; detector centre indices are in pixels. Distances, ``step_size``,
-    ``voxel_size`` and ROI offsets share one physical length unit.
+    Geometry of view a: the source sits at (-sod sin b + src_x, sod cos b + src_y, src_z) with b
+    = angles[a]; the detector plane stays at distance sod - sid from the rotation axis on the
+    far side (src_* move the source only), and pixel (u, v) is at ((u - det_u0) pixel_size,
+    (v - det_v0) pixel_size) in that plane.  ``volume`` is [D, H, W] (z, y, x); voxel (i, j, k)
+    has its centre at ((k - W/2) voxel_size + off_x, (j - H/2) voxel_size + off_y,
+    (i - D/2) voxel_size + off_z).  Each ray samples the trilinearly interpolated volume every
+    ``step_size`` from the point where it enters the volume box, and the projection is the sum
+    of the samples times ``step_size``.
 
-    The ROI uses x = (index_x - W/2) * voxel_size + off_x, and similarly for
-    y and z. The integrator samples the trilinear volume at fixed physical
-    steps from the box entry point. Its first-order gradients are piecewise
-    derivatives: sample-count, interpolation-cell and box-intersection changes
-    are nonsmooth. Scalar settings and ROI offsets have no gradients; higher
-    derivatives are not supported. This ray integrator and the weighted FDK
-    backprojector share geometry but are not a strict adjoint pair.
+    All nine tensors are float32 on one GPU (geometry tensors: shape [A]); lengths share one
+    unit.  The gradients are first-order and piecewise: they ignore the jumps where a ray gains
+    or loses a sample, crosses into another interpolation cell or changes its entry face.
+    Scalars and ROI offsets get no gradient.  This projector's transpose is its own adjoint,
+    not ``backproject``, which is distance-weighted for FDK.
     """
     return DifferentiableSourceOnlyForwardProjector.apply(
-        volume, angles, sid, sod, det_u0_arr, det_v0_arr,
-        src_x_arr, src_y_arr, src_z_arr, pixel_size,
-        detector_u, detector_v, step_size, voxel_size,
-        off_x, off_y, off_z,
-    )
+        volume, angles, sid, sod, det_u0_arr, det_v0_arr, src_x_arr, src_y_arr, src_z_arr,
+        pixel_size, detector_u, detector_v, step_size, voxel_size, off_x, off_y, off_z)

@@ -422,6 +422,8 @@ static int check_bp_common(int64_t A, int64_t U, int64_t V, int64_t D, int64_t H
     CBCT_CHECK_ARG(D > 0 && H > 0 && W > 0 && D <= INT32_MAX && H <= INT32_MAX && W <= INT32_MAX,
                    "D,H,W must be positive and fit in a 32-bit integer");
     CBCT_CHECK_ARG(D <= INT32_MAX / H && D * H <= INT32_MAX / W, "volume exceeds the kernel's 32-bit indexing limit");
+    // z is covered by workgroups of 64 voxels (or more) along the launch grid's second dimension
+    CBCT_CHECK_ARG(D <= 64 * 65535, "D exceeds the backprojector's launch limit of 4194240 voxels");
     int st;
     if ((st = check_positive_finite(pix, "pixel_size"))) return st;
     if ((st = check_positive_finite(vs, "voxel_size"))) return st;
@@ -484,14 +486,20 @@ extern "C" size_t cbct_bp_workspace_bytes(int64_t A, int64_t D, int64_t H, int64
 
 // frames | per-workgroup geometry partials [A][blocks][8] | (deterministic) int64 accumulator
 // [A,U,V] | max bits + per-view exponents
-extern "C" size_t cbct_bp_backward_workspace_bytes(int64_t A, int64_t U, int64_t V, int64_t D, int64_t H,
-                                                   int64_t W) {
+extern "C" size_t cbct_bp_backward_workspace_bytes_ex(int64_t A, int64_t U, int64_t V, int64_t D, int64_t H,
+                                                      int64_t W, const cbct_fp_options* options) {
     if (A < 1 || U < 1 || V < 1 || D < 1 || H < 1 || W < 1) return 0;
+    int det = 0;
+    if (call_deterministic(options, &det)) return 0;
     const dim3 grid = bp_bwd_grid(D, H, W);
     size_t n = align256((size_t)A * sizeof(ViewFrame)) + align256((size_t)A * grid.x * grid.y * 8 * sizeof(float));
-    if (cbct_fp_get_deterministic())
-        n += align256((size_t)A * U * V * sizeof(unsigned long long)) + align256(sizeof(unsigned) * (1 + A));
+    if (det) n += align256((size_t)A * U * V * sizeof(unsigned long long)) + align256(sizeof(unsigned) * (1 + A));
     return n;
+}
+
+extern "C" size_t cbct_bp_backward_workspace_bytes(int64_t A, int64_t U, int64_t V, int64_t D, int64_t H,
+                                                   int64_t W) {
+    return cbct_bp_backward_workspace_bytes_ex(A, U, V, D, H, W, nullptr);
 }
 
 extern "C" int cbct_bp_forward(const float* projections, int64_t A, int64_t U, int64_t V,
@@ -516,24 +524,27 @@ extern "C" int cbct_bp_forward(const float* projections, int64_t A, int64_t U, i
     return launch_bp_forward(projections, frames, g, (int)A, 1.0f, 0, volume, stream);
 }
 
-extern "C" int cbct_bp_backward(const float* grad_volume, const float* projections,
-                                int64_t A, int64_t U, int64_t V,
-                                const float* angles, const float* sid, const float* sod,
-                                const float* det_u0, const float* det_v0,
-                                const float* src_x, const float* src_y, const float* src_z,
-                                float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
-                                float off_x, float off_y, float off_z,
-                                float* grad_projections, float* grad_geometry,
-                                void* workspace, size_t workspace_bytes, void* stream_) {
+extern "C" int cbct_bp_backward_ex(const float* grad_volume, const float* projections,
+                                   int64_t A, int64_t U, int64_t V,
+                                   const float* angles, const float* sid, const float* sod,
+                                   const float* det_u0, const float* det_v0,
+                                   const float* src_x, const float* src_y, const float* src_z,
+                                   float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
+                                   float off_x, float off_y, float off_z,
+                                   float* grad_projections, float* grad_geometry, const cbct_fp_options* options,
+                                   void* workspace, size_t workspace_bytes, void* stream_) {
     int st;
     if ((st = check_bp_common(A, U, V, D, H, W, pixel_size, voxel_size, off_x, off_y, off_z))) return st;
     const float* gg[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
     if ((st = check_geometry(gg, A))) return st;
     CBCT_CHECK_ARG(grad_volume != nullptr && projections != nullptr && grad_projections != nullptr &&
                    grad_geometry != nullptr, "gradient and projection pointers must be non-null");
-    CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= cbct_bp_backward_workspace_bytes(A, U, V, D, H, W),
+    int det_flag = 0;
+    if ((st = call_deterministic(options, &det_flag))) return st;
+    CBCT_CHECK_ARG(workspace != nullptr &&
+                   workspace_bytes >= cbct_bp_backward_workspace_bytes_ex(A, U, V, D, H, W, options),
                    "workspace too small for cbct_bp_backward (see cbct_bp_backward_workspace_bytes)");
-    const bool det = cbct_fp_get_deterministic() != 0;
+    const bool det = det_flag != 0;
     hipStream_t stream = (hipStream_t)stream_;
     const dim3 grid = bp_bwd_grid(D, H, W);
     const int64_t nblk = (int64_t)grid.x * grid.y;
@@ -579,4 +590,18 @@ extern "C" int cbct_bp_backward(const float* grad_volume, const float* projectio
         CBCT_LAUNCHED("deterministic projection gradient conversion");
     }
     return launch_reduce_geometry(partials, A, nblk, grad_geometry, stream);
+}
+
+extern "C" int cbct_bp_backward(const float* grad_volume, const float* projections,
+                                int64_t A, int64_t U, int64_t V,
+                                const float* angles, const float* sid, const float* sod,
+                                const float* det_u0, const float* det_v0,
+                                const float* src_x, const float* src_y, const float* src_z,
+                                float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
+                                float off_x, float off_y, float off_z,
+                                float* grad_projections, float* grad_geometry,
+                                void* workspace, size_t workspace_bytes, void* stream_) {
+    return cbct_bp_backward_ex(grad_volume, projections, A, U, V, angles, sid, sod, det_u0, det_v0, src_x, src_y,
+                               src_z, pixel_size, D, H, W, voxel_size, off_x, off_y, off_z, grad_projections,
+                               grad_geometry, nullptr, workspace, workspace_bytes, stream_);
 }

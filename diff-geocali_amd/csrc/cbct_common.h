@@ -372,5 +372,8 @@ int launch_fill(float* p, int64_t n, float value, hipStream_t stream);
 int launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t stream);
 int launch_absmax(const float* x, int64_t n, unsigned* maxbits, hipStream_t stream);
 int launch_reduce_geometry(const float* partials, int64_t A, int64_t tiles, float* ggeom, hipStream_t stream);
+// The deterministic setting of a call: the caller's options, or the process-wide default
+// (cbct_fp.hip).
+int call_deterministic(const cbct_fp_options* opt, int* det);
 
 }  // namespace cbct

@@ -291,6 +291,7 @@ extern "C" int cbct_fdk_reconstruct_ex(const float* projections, int64_t A, int6
     CBCT_CHECK_ARG(angle_chunk_size >= 1, "angle_chunk_size must be at least 1");
     CBCT_CHECK_ARG(D > 0 && H > 0 && W > 0 && D * H * W <= INT32_MAX,
                    "volume_shape must be positive and within the 2^31-1 element limit");
+    CBCT_CHECK_ARG(D <= 64 * 65535, "D exceeds the backprojector's launch limit of 4194240 voxels");
     CBCT_CHECK_ARG(!short_scan || A >= 2, "a short-scan reconstruction needs at least two views");
     if ((st = check_positive_finite(voxel_size, "voxel_size"))) return st;
     if ((st = check_finite(off_x, "off_x")) || (st = check_finite(off_y, "off_y")) ||

@@ -14,6 +14,7 @@
 #include "cbct_fp_slab.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace cbct {
 
@@ -222,8 +223,50 @@ struct FpTuning {
     int cap;              // LDS floats per staged box
     int layout;           // lane layout of a tile
 };
+// The process-wide defaults (cbct_fp_set_tuning_op, cbct_fp_set_deterministic), read by the entry
+// points without options and by the *_ex entry points given options == NULL.  The mutex keeps a
+// concurrent set from tearing a read; a call with options never reads them.
+static std::mutex g_defaults_mutex;
 static FpTuning g_tuning_op[2] = {{1, 12, 5120, 0}, {1, 12, 3840, 1}};
 static int g_deterministic = 0;   // volume gradient through order-independent int64 accumulation
+
+struct FpCall {
+    FpTuning t;
+    int det;
+};
+
+static int check_options(const cbct_fp_options& o) {
+    CBCT_CHECK_ARG(o.use_slab == 0 || o.use_slab == 1, "use_slab must be 0 or 1");
+    CBCT_CHECK_ARG(o.slab_thickness >= 1 && o.slab_thickness <= 64, "slab_thickness must be in [1, 64]");
+    CBCT_CHECK_ARG(o.lds_floats >= 256 && o.lds_floats <= 8192, "lds_floats must be in [256, 8192]");
+    CBCT_CHECK_ARG(o.layout == 0 || o.layout == 1, "layout must be 0 or 1");
+    CBCT_CHECK_ARG(o.deterministic == 0 || o.deterministic == 1, "deterministic must be 0 or 1");
+    return CBCT_OK;
+}
+
+// The settings of one call in operation class op (0: gathering kernels, 1: adjoint with the
+// volume gradient): the caller's options, or the process-wide defaults.
+static int call_settings(const cbct_fp_options* opt, int op, FpCall& c) {
+    if (opt) {
+        const int st = check_options(*opt);
+        if (st) return st;
+        c.t = FpTuning{opt->use_slab, opt->slab_thickness, opt->lds_floats, opt->layout};
+        c.det = opt->deterministic;
+        return CBCT_OK;
+    }
+    std::lock_guard<std::mutex> lock(g_defaults_mutex);
+    c.t = g_tuning_op[op];
+    c.det = g_deterministic;
+    return CBCT_OK;
+}
+
+int call_deterministic(const cbct_fp_options* opt, int* det) {
+    FpCall c;
+    const int st = call_settings(opt, 0, c);
+    if (st) return st;
+    *det = c.det;
+    return CBCT_OK;
+}
 
 // ---- deterministic volume gradient helpers
 __global__ void zero_u64_kernel(unsigned long long* __restrict__ p, int64_t n) {
@@ -324,31 +367,28 @@ extern "C" size_t cbct_fp_workspace_bytes(int64_t A, int64_t U, int64_t V) {
     return align256((size_t)A * sizeof(ViewFrame)) + (size_t)A * fp_tiles(U, V) * 8 * sizeof(float);
 }
 
-extern "C" int cbct_fp_forward(const float* volume, int64_t D, int64_t H, int64_t W,
-                               const float* angles, const float* sid, const float* sod,
-                               const float* det_u0, const float* det_v0,
-                               const float* src_x, const float* src_y, const float* src_z, int64_t A,
-                               float pixel_size, int64_t U, int64_t V, float step_size,
-                               float voxel_size, float off_x, float off_y, float off_z,
-                               float* projections, void* workspace, size_t workspace_bytes,
-                               void* stream_) {
+static int fp_forward_impl(const float* volume, int64_t D, int64_t H, int64_t W, const float* const g[8],
+                           int64_t A, float pixel_size, int64_t U, int64_t V, float step_size, float voxel_size,
+                           float off_x, float off_y, float off_z, float* projections, const cbct_fp_options* opt,
+                           void* workspace, size_t workspace_bytes, void* stream_) {
     int st;
     if ((st = check_fp_common(D, H, W, A, U, V, pixel_size, step_size, voxel_size, off_x, off_y, off_z))) return st;
-    const float* g[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
     if ((st = check_geometry(g, A))) return st;
     CBCT_CHECK_ARG(volume != nullptr && projections != nullptr, "volume and projections must be non-null");
+    FpCall call;
+    if ((st = call_settings(opt, 0, call))) return st;
     CBCT_CHECK_ARG(workspace != nullptr && workspace_bytes >= cbct_fp_workspace_bytes(A, U, V),
                    "workspace too small for cbct_fp_forward");
     hipStream_t stream = (hipStream_t)stream_;
     ViewFrame* frames = (ViewFrame*)workspace;
-    const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    const GeomPtrs gp{g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7]};
     const FrameGuard guard = make_guard_host(D, H, W, voxel_size, off_x, off_y, off_z, U, V, pixel_size, step_size);
     if ((st = launch_frames(gp, A, frames, stream, &guard))) return st;
     const Box b = make_box_host(D, H, W, voxel_size, off_x, off_y, off_z);
     const int tu = (int)ceil_div(U, FP_TILE), tv = (int)ceil_div(V, FP_TILE);
     const int nblocks = (int)(A * tu * tv);
     const float inv_vs = 1.0f / voxel_size;
-    const FpTuning& tf = g_tuning_op[0];
+    const FpTuning& tf = call.t;
     if (tf.use_slab) {
         const SlabArgs args{volume, nullptr, frames, projections, nullptr, nullptr, b, (int)U, (int)V, tu, tv,
                             nblocks, pixel_size, step_size, voxel_size, inv_vs, tf.T, tf.cap,
@@ -367,44 +407,80 @@ extern "C" int cbct_fp_forward(const float* volume, int64_t D, int64_t H, int64_
     return CBCT_OK;
 }
 
+extern "C" int cbct_fp_forward(const float* volume, int64_t D, int64_t H, int64_t W,
+                               const float* angles, const float* sid, const float* sod,
+                               const float* det_u0, const float* det_v0,
+                               const float* src_x, const float* src_y, const float* src_z, int64_t A,
+                               float pixel_size, int64_t U, int64_t V, float step_size,
+                               float voxel_size, float off_x, float off_y, float off_z,
+                               float* projections, void* workspace, size_t workspace_bytes,
+                               void* stream_) {
+    const float* g[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    return fp_forward_impl(volume, D, H, W, g, A, pixel_size, U, V, step_size, voxel_size, off_x, off_y, off_z,
+                           projections, nullptr, workspace, workspace_bytes, stream_);
+}
+
+extern "C" int cbct_fp_forward_ex(const float* volume, int64_t D, int64_t H, int64_t W,
+                                  const float* angles, const float* sid, const float* sod,
+                                  const float* det_u0, const float* det_v0,
+                                  const float* src_x, const float* src_y, const float* src_z, int64_t A,
+                                  float pixel_size, int64_t U, int64_t V, float step_size,
+                                  float voxel_size, float off_x, float off_y, float off_z,
+                                  float* projections, const cbct_fp_options* options, void* workspace,
+                                  size_t workspace_bytes, void* stream_) {
+    const float* g[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    return fp_forward_impl(volume, D, H, W, g, A, pixel_size, U, V, step_size, voxel_size, off_x, off_y, off_z,
+                           projections, options, workspace, workspace_bytes, stream_);
+}
+
 static size_t det_extra_bytes(int64_t D, int64_t H, int64_t W) {
     return align256((size_t)D * H * W * sizeof(unsigned long long)) + 256;
 }
 
+extern "C" size_t cbct_fp_backward_workspace_bytes_ex(int64_t D, int64_t H, int64_t W, int64_t A, int64_t U,
+                                                      int64_t V, const cbct_fp_options* options) {
+    if (D <= 0 || H <= 0 || W <= 0) return 0;
+    FpCall call;
+    if (call_settings(options, 1, call)) return 0;
+    return cbct_fp_workspace_bytes(A, U, V) + (call.det ? det_extra_bytes(D, H, W) : 0);
+}
+
 extern "C" size_t cbct_fp_backward_workspace_bytes(int64_t D, int64_t H, int64_t W, int64_t A, int64_t U,
                                                    int64_t V) {
-    if (D <= 0 || H <= 0 || W <= 0) return 0;
-    return cbct_fp_workspace_bytes(A, U, V) + (g_deterministic ? det_extra_bytes(D, H, W) : 0);
+    return cbct_fp_backward_workspace_bytes_ex(D, H, W, A, U, V, nullptr);
 }
 
 extern "C" int cbct_fp_set_deterministic(int on) {
     CBCT_CHECK_ARG(on == 0 || on == 1, "deterministic must be 0 or 1");
+    std::lock_guard<std::mutex> lock(g_defaults_mutex);
     g_deterministic = on;
     return CBCT_OK;
 }
 
-extern "C" int cbct_fp_get_deterministic(void) { return g_deterministic; }
+extern "C" int cbct_fp_get_deterministic(void) {
+    std::lock_guard<std::mutex> lock(g_defaults_mutex);
+    return g_deterministic;
+}
 
-extern "C" int cbct_fp_backward(const float* grad_projection, const float* volume,
-                                int64_t D, int64_t H, int64_t W,
-                                const float* angles, const float* sid, const float* sod,
-                                const float* det_u0, const float* det_v0,
-                                const float* src_x, const float* src_y, const float* src_z,
-                                int64_t A, int64_t U, int64_t V,
-                                float pixel_size, float step_size, float voxel_size,
-                                int compute_grad_volume, float off_x, float off_y, float off_z,
-                                float* grad_volume, float* grad_geometry,
-                                void* workspace, size_t workspace_bytes, void* stream_) {
+static int fp_backward_impl(const float* grad_projection, const float* volume, int64_t D, int64_t H, int64_t W,
+                            const float* const g[8], int64_t A, int64_t U, int64_t V, float pixel_size,
+                            float step_size, float voxel_size, int compute_grad_volume, float off_x, float off_y,
+                            float off_z, float* grad_volume, float* grad_geometry, bool geometry_optional,
+                            const cbct_fp_options* opt, void* workspace, size_t workspace_bytes, void* stream_) {
     int st;
     if ((st = check_fp_common(D, H, W, A, U, V, pixel_size, step_size, voxel_size, off_x, off_y, off_z))) return st;
-    const float* g[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
     if ((st = check_geometry(g, A))) return st;
-    CBCT_CHECK_ARG(grad_projection != nullptr && volume != nullptr && grad_geometry != nullptr,
-                   "grad_projection, volume and grad_geometry must be non-null");
+    CBCT_CHECK_ARG(grad_projection != nullptr && volume != nullptr, "grad_projection and volume must be non-null");
+    CBCT_CHECK_ARG(geometry_optional || grad_geometry != nullptr, "grad_geometry must be non-null");
     CBCT_CHECK_ARG(!compute_grad_volume || grad_volume != nullptr,
                    "grad_volume must be non-null when compute_grad_volume is set");
-    const bool det = g_deterministic && compute_grad_volume;
-    const FpTuning& tb = g_tuning_op[compute_grad_volume ? 1 : 0];
+    const bool geometry = grad_geometry != nullptr;
+    CBCT_CHECK_ARG(compute_grad_volume || geometry,
+                   "nothing to compute: neither the volume gradient nor the geometry gradients are requested");
+    FpCall call;
+    if ((st = call_settings(opt, compute_grad_volume ? 1 : 0, call))) return st;
+    const bool det = call.det && compute_grad_volume;
+    const FpTuning& tb = call.t;
     CBCT_CHECK_ARG(!det || tb.use_slab, "the deterministic volume gradient needs the slab kernels");
     const size_t base_bytes = cbct_fp_workspace_bytes(A, U, V);
     CBCT_CHECK_ARG(workspace != nullptr &&
@@ -413,7 +489,7 @@ extern "C" int cbct_fp_backward(const float* grad_projection, const float* volum
     hipStream_t stream = (hipStream_t)stream_;
     ViewFrame* frames = (ViewFrame*)workspace;
     float* partials = (float*)((char*)workspace + align256((size_t)A * sizeof(ViewFrame)));
-    const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    const GeomPtrs gp{g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7]};
     const FrameGuard guard = make_guard_host(D, H, W, voxel_size, off_x, off_y, off_z, U, V, pixel_size, step_size);
     if ((st = launch_frames(gp, A, frames, stream, &guard))) return st;
     unsigned long long* gacc = nullptr;
@@ -444,24 +520,58 @@ extern "C" int cbct_fp_backward(const float* grad_projection, const float* volum
                       inv_vs, tb.T, tb.cap, tb.layout};
         args.gacc = gacc;
         args.gexp = gexp;
-        if ((st = launch_fp_slab(compute_grad_volume ? 2 : 1, false, args, stream))) return st;
+        if ((st = launch_fp_slab(compute_grad_volume ? 2 : 1, false, args, stream, geometry))) return st;
         if (det) {
             det_convert_kernel<<<(unsigned)ceil_div(nvox, (int64_t)256), 256, 0, stream>>>(gacc, nvox, gexp,
                                                                                          grad_volume);
             CBCT_LAUNCHED("deterministic volume gradient conversion");
         }
-    } else if (compute_grad_volume)
+    } else if (compute_grad_volume) {
         fp_backward_kernel<true><<<nblocks, FP_THREADS, 0, stream>>>(
             grad_projection, volume, b, frames, (int)U, (int)V, tu, tv, nblocks, pixel_size,
             step_size, inv_vs, grad_volume, partials);
-    else
+    } else {
         fp_backward_kernel<false><<<nblocks, FP_THREADS, 0, stream>>>(
             grad_projection, volume, b, frames, (int)U, (int)V, tu, tv, nblocks, pixel_size,
             step_size, inv_vs, nullptr, partials);
+    }
     CBCT_LAUNCHED("source-only backward kernel launch");
+    if (!geometry) return CBCT_OK;
     fp_reduce_geometry<<<(unsigned)A, 256, 0, stream>>>(partials, (int)A, tu * tv, grad_geometry);
     CBCT_LAUNCHED("geometry reduction kernel launch");
     return CBCT_OK;
+}
+
+extern "C" int cbct_fp_backward(const float* grad_projection, const float* volume,
+                                int64_t D, int64_t H, int64_t W,
+                                const float* angles, const float* sid, const float* sod,
+                                const float* det_u0, const float* det_v0,
+                                const float* src_x, const float* src_y, const float* src_z,
+                                int64_t A, int64_t U, int64_t V,
+                                float pixel_size, float step_size, float voxel_size,
+                                int compute_grad_volume, float off_x, float off_y, float off_z,
+                                float* grad_volume, float* grad_geometry,
+                                void* workspace, size_t workspace_bytes, void* stream_) {
+    const float* g[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    return fp_backward_impl(grad_projection, volume, D, H, W, g, A, U, V, pixel_size, step_size, voxel_size,
+                            compute_grad_volume, off_x, off_y, off_z, grad_volume, grad_geometry, false, nullptr,
+                            workspace, workspace_bytes, stream_);
+}
+
+extern "C" int cbct_fp_backward_ex(const float* grad_projection, const float* volume,
+                                   int64_t D, int64_t H, int64_t W,
+                                   const float* angles, const float* sid, const float* sod,
+                                   const float* det_u0, const float* det_v0,
+                                   const float* src_x, const float* src_y, const float* src_z,
+                                   int64_t A, int64_t U, int64_t V,
+                                   float pixel_size, float step_size, float voxel_size,
+                                   int compute_grad_volume, float off_x, float off_y, float off_z,
+                                   float* grad_volume, float* grad_geometry, const cbct_fp_options* options,
+                                   void* workspace, size_t workspace_bytes, void* stream_) {
+    const float* g[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    return fp_backward_impl(grad_projection, volume, D, H, W, g, A, U, V, pixel_size, step_size, voxel_size,
+                            compute_grad_volume, off_x, off_y, off_z, grad_volume, grad_geometry, true, options,
+                            workspace, workspace_bytes, stream_);
 }
 
 extern "C" int cbct_fp_count_samples(int64_t D, int64_t H, int64_t W,
@@ -496,10 +606,10 @@ extern "C" int cbct_fp_count_samples(int64_t D, int64_t H, int64_t W,
 
 extern "C" int cbct_fp_set_tuning_op(int op, int use_slab, int slab_thickness, int lds_floats, int layout) {
     CBCT_CHECK_ARG(op == 0 || op == 1, "op must be 0 (gathering kernels) or 1 (volume-gradient adjoint)");
-    CBCT_CHECK_ARG(use_slab == 0 || use_slab == 1, "use_slab must be 0 or 1");
-    CBCT_CHECK_ARG(slab_thickness >= 1 && slab_thickness <= 64, "slab_thickness must be in [1, 64]");
-    CBCT_CHECK_ARG(lds_floats >= 256 && lds_floats <= 8192, "lds_floats must be in [256, 8192]");
-    CBCT_CHECK_ARG(layout == 0 || layout == 1, "layout must be 0 or 1");
+    const cbct_fp_options o{use_slab, slab_thickness, lds_floats, layout, 0};
+    const int st = check_options(o);
+    if (st) return st;
+    std::lock_guard<std::mutex> lock(g_defaults_mutex);
     g_tuning_op[op] = FpTuning{use_slab, slab_thickness, lds_floats, layout};
     return CBCT_OK;
 }
@@ -507,6 +617,7 @@ extern "C" int cbct_fp_set_tuning_op(int op, int use_slab, int slab_thickness, i
 extern "C" int cbct_fp_get_tuning_op(int op, int* out4) {
     CBCT_CHECK_ARG(op == 0 || op == 1, "op must be 0 (gathering kernels) or 1 (volume-gradient adjoint)");
     CBCT_CHECK_ARG(out4 != nullptr, "out4 must be non-null");
+    std::lock_guard<std::mutex> lock(g_defaults_mutex);
     const FpTuning& t = g_tuning_op[op];
     out4[0] = t.use_slab;
     out4[1] = t.T;
@@ -522,3 +633,12 @@ extern "C" int cbct_fp_set_tuning(int use_slab, int slab_thickness, int lds_floa
 }
 
 extern "C" int cbct_fp_get_tuning(int* out4) { return cbct_fp_get_tuning_op(0, out4); }
+
+extern "C" int cbct_fp_default_options(int op, cbct_fp_options* out) {
+    CBCT_CHECK_ARG(op == 0 || op == 1, "op must be 0 (gathering kernels) or 1 (volume-gradient adjoint)");
+    CBCT_CHECK_ARG(out != nullptr, "out must be non-null");
+    FpCall c;
+    call_settings(nullptr, op, c);
+    *out = cbct_fp_options{c.t.use_slab, c.t.T, c.t.cap, c.t.layout, c.det};
+    return CBCT_OK;
+}

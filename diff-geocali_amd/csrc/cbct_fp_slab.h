@@ -49,8 +49,9 @@ struct CalibDev {
 constexpr int LM_NE = 45;               // 36 (upper triangle of J^T J) + 8 (J^T r) + loss
 constexpr int LM_STATE = 8 + LM_NE + 2; // geometry, normal equations, lambda, loss
 
-// mode 0: forward; 1: adjoint (geometry only); 2: adjoint with the volume gradient.
-int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream);
+// mode 0: forward; 1: adjoint (geometry only); 2: adjoint with the volume gradient, and with the
+// eight geometry gradients unless geometry is false.
+int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream, bool geometry = true);
 
 // Mode 3 (fused forward + squared residual + geometry adjoint; solver 0) or mode 4 (normal
 // equations; solver 1) over every tile of every problem, persistent workgroups fed by the work

@@ -42,13 +42,18 @@ __device__ __forceinline__ void pixel_dir(const ViewFrame& f, float u, float v, 
 // 4: Gauss-Newton step -- as 3, but each ray's eight geometry derivatives j of its projection and
 //    its residual r give the tile's normal equations: the 36 entries of sum j j^T (upper triangle,
 //    row-major), the 8 of sum j r and the loss (45 partials per tile).
-template <int MODE, bool DIV, bool ZOFF = false, int CAPC = 0>
+// GEO (MODE 2 only): with false, the geometry gradients are not wanted -- the adjoint then needs
+// no volume taps at all (the scatter depends only on the sample positions and the upstream
+// weights): no staged volume box, no interpolation, no geometry sums, one LDS box per workgroup.
+template <int MODE, bool DIV, bool ZOFF = false, int CAPC = 0, bool GEO = true>
 __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
+    static_assert(GEO || MODE == 2, "only the volume-gradient adjoint can skip the geometry");
+    constexpr bool TAPS = GEO;        // the march reads volume taps
     extern __shared__ float lds[];
     float* vbox = lds;
     // CAPC > 0: the LDS capacity as a compile-time constant (== P.cap), so the gradient box's
     // addresses are the tap addresses plus an immediate offset
-    float* gbox = lds + (CAPC > 0 ? CAPC : P.cap);
+    float* gbox = TAPS ? lds + (CAPC > 0 ? CAPC : P.cap) : lds;
     const Box& b = P.b;
 
     // ---- tile and ray of this lane
@@ -156,6 +161,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     // Everything a sample does once its eight taps are known.  Lanes with go == false (no sample in
     // the current slab) leave every sum unchanged.
     auto use_taps = [&](const Taps& tp, float fx, float fy, float fz, bool go) {
+        if (!TAPS) return;
         float val, sgx, sgy, sgz;
         tri_eval(tp, fx, fy, fz, val, sgx, sgy, sgz);
         if (MODE == 0 || MODE >= 3) integral += go ? val : 0.f;
@@ -175,7 +181,8 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         const Cell c = cell_of(b, x, y, z);
         const int i000 = (c.z0 * b.H + c.y0) * b.W + c.x0;
         const int ox1 = c.sx, oy1 = c.sy * b.W, oz1 = c.sz * plane;
-        const Taps tp = load_taps_strided(P.vol, i000, ox1, oy1, oz1);
+        Taps tp{};
+        if (TAPS) tp = load_taps_strided(P.vol, i000, ox1, oy1, oz1);
         if (MODE == 2 && P.gacc) {
             const Weights w = tri_weights(c.fx, c.fy, c.fz);
             unsigned long long* q = P.gacc + i000;
@@ -382,9 +389,11 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 // lanes skip the block.)  Byte offset of the cell in the box: 24-bit products (cell
                 // coordinates and box strides are small), full rate.
                 const unsigned idx4 = (unsigned)(__mul24(zr, PL4) + __mul24(yr, B04) + 4 * xr - base4);
-                const Taps tp = load_taps_strided(reinterpret_cast<const float*>(
-                                                      reinterpret_cast<const char*>(vbox) + idx4), 0, 1, B0, PL);
-                use_taps(tp, fx, fy, fz, true);
+                if (TAPS) {
+                    const Taps tp = load_taps_strided(reinterpret_cast<const float*>(
+                                                          reinterpret_cast<const char*>(vbox) + idx4), 0, 1, B0, PL);
+                    use_taps(tp, fx, fy, fz, true);
+                }
                 if (MODE == 2) {
                     // the scaled trilinear weights as a z, then y split of gsq, then x weights
                     // plus half a unit of gsq's sign (every weight has it), so that truncation
@@ -511,13 +520,13 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
 #pragma unroll
                     for (int k = 0; k < NB; ++k) {
                         const int i = i0 + k * SL_THREADS;
-                        vv[k] = i < nbox ? P.vol[box_voxel(i)] : 0.f;
+                        vv[k] = TAPS && i < nbox ? P.vol[box_voxel(i)] : 0.f;
                     }
 #pragma unroll
                     for (int k = 0; k < NB; ++k) {
                         const int i = i0 + k * SL_THREADS;
                         if (i < nbox) {
-                            vbox[i] = vv[k];
+                            if (TAPS) vbox[i] = vv[k];
                             if (MODE == 2) gboxu[i] = 0u;
                         }
                     }
@@ -564,6 +573,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
         if (inr) P.proj[((int64_t)a * P.U + u) * P.V + v] = hit ? integral * P.step : 0.f;
         return;
     }
+    if (!GEO) return;
     {
         // the constant factors of the geometry sums: upstream weight * step (1 in the calibration
         // modes, whose weight comes below) and 1/voxel size; sum over samples of grad . d
@@ -658,14 +668,14 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     }
 }
 
-template <int MODE, bool DIV, bool ZOFF, int CAPC = 0>
+template <int MODE, bool DIV, bool ZOFF, int CAPC = 0, bool GEO = true>
 __global__ __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 5 : 1)))
 void fp_slab_kernel(SlabArgs P) {
     // Workgroups are dispatched round-robin over the 8 XCDs; permute so that consecutive tiles of a
     // view share an XCD (and its L2).
     const int xcd = blockIdx.x & 7;
     const int tile = ((P.nblocks & 7) == 0) ? xcd * (P.nblocks >> 3) + (blockIdx.x >> 3) : (int)blockIdx.x;
-    slab_tile<MODE, DIV, ZOFF, CAPC>(P, tile);
+    slab_tile<MODE, DIV, ZOFF, CAPC, GEO>(P, tile);
 }
 
 // Persistent calibration kernel: workgroups pull tiles of all problems from one counter.
@@ -705,11 +715,16 @@ int launch_fp_slab_batched(const CalibDev* probs, int nprob, int total_tiles, in
     return CBCT_OK;
 }
 
-int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream) {
-    const size_t shmem = (size_t)args.cap * sizeof(float) * (mode == 2 ? 2 : 1);
+int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream, bool geometry) {
+    const size_t shmem = (size_t)args.cap * sizeof(float) * (mode == 2 && geometry ? 2 : 1);
     const dim3 grid((unsigned)args.nblocks), block(SL_THREADS);
     const bool zoff = args.b.offx == 0.f && args.b.offy == 0.f && args.b.offz == 0.f;
-    if (mode == 0) {
+    if (mode == 2 && !geometry) {
+        // volume gradient only: no taps, so neither the LDS capacity nor the offsets need to be
+        // compile-time constants
+        if (zoff) fp_slab_kernel<2, false, true, 0, false><<<grid, block, shmem, stream>>>(args);
+        else fp_slab_kernel<2, false, false, 0, false><<<grid, block, shmem, stream>>>(args);
+    } else if (mode == 0) {
         if (div) {
             if (zoff) fp_slab_kernel<0, true, true><<<grid, block, shmem, stream>>>(args);
             else fp_slab_kernel<0, true, false><<<grid, block, shmem, stream>>>(args);

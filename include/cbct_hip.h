@@ -40,6 +40,27 @@ const char* cbct_version(void);
 /* Message of the last failing call on this thread ("" if none). */
 const char* cbct_last_error(void);
 
+/* ---- per-call options ----------------------------------------------------
+ * No reference counterpart (the reference has one kernel per operator and no modes).
+ * Kernel selection and the deterministic mode can be given per call through the *_ex
+ * entry points, which then read no process-wide state: calls on different host threads or
+ * streams with different options do not affect each other.  options == NULL means the
+ * process-wide defaults of cbct_fp_set_tuning_op / cbct_fp_set_deterministic (which the
+ * entry points without _ex always use); cbct_fp_default_options reads them.  Every setting
+ * gives the same projections bit for bit; see cbct_fp_set_tuning_op for the fields. */
+typedef struct {
+    int use_slab;        /* 1: slab-staged kernels; 0: direct kernels (one ray per lane, global taps) */
+    int slab_thickness;  /* voxel planes per staged slab, 1..64 */
+    int lds_floats;      /* LDS floats of one staged box, 256..8192 */
+    int layout;          /* lanes of a 16x16 ray tile: 0 = 8x8 blocks per wave, 1 = interleaved, stride 2 */
+    int deterministic;   /* 1: int64 fixed-point volume gradient (projector adjoint) and projection
+                            gradient (backprojection backward): bitwise reproducible */
+} cbct_fp_options;
+
+/* The process-wide defaults of operation class op (0: kernels that only gather -- forward,
+ * geometry-only adjoint, calibration march; 1: adjoint with the volume gradient). */
+int cbct_fp_default_options(int op, cbct_fp_options* out);
+
 /* ---- forward projector: src_cbct_forwardprojector_cuda ----------------- */
 
 /* Workspace bytes needed by cbct_fp_forward / cbct_fp_backward. */
@@ -56,12 +77,23 @@ int cbct_fp_forward(const float* volume, int64_t D, int64_t H, int64_t W,
                     float off_x, float off_y, float off_z,
                     float* projections, void* workspace, size_t workspace_bytes, void* stream);
 
+/* cbct_fp_forward with per-call options (NULL: the process-wide defaults).  Replaces the same
+ * forward_cuda, csrc/src_cbct_forwardprojector_cuda.cu:566-631. */
+int cbct_fp_forward_ex(const float* volume, int64_t D, int64_t H, int64_t W,
+                       const float* angles, const float* sid, const float* sod,
+                       const float* det_u0, const float* det_v0,
+                       const float* src_x, const float* src_y, const float* src_z, int64_t A,
+                       float pixel_size, int64_t U, int64_t V, float step_size, float voxel_size,
+                       float off_x, float off_y, float off_z,
+                       float* projections, const cbct_fp_options* options,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
 /* Workspace of cbct_fp_backward: cbct_fp_workspace_bytes plus, in deterministic
  * mode, 8 bytes per voxel (+256) for the integer accumulator.  Query it with the
  * deterministic setting that the call will use. */
 size_t cbct_fp_backward_workspace_bytes(int64_t D, int64_t H, int64_t W, int64_t A, int64_t U, int64_t V);
 
-/* Deterministic mode (process-wide, default off): the projector adjoint's volume
+/* Deterministic mode (process-wide default, off; per call: cbct_fp_options): the projector adjoint's volume
  * gradient is accumulated as int64 fixed-point values (scale 2^-E with E chosen
  * from max|grad * step|, so each sample keeps >= 40 bits), which makes the result
  * bitwise reproducible run to run; the reference's float atomics are not
@@ -86,6 +118,26 @@ int cbct_fp_backward(const float* grad_projection, const float* volume,
                      float* grad_volume, float* grad_geometry,
                      void* workspace, size_t workspace_bytes, void* stream);
 
+/* cbct_fp_backward with per-call options (NULL: the process-wide defaults), whose
+ * workspace is cbct_fp_backward_workspace_bytes_ex with the same options.  grad_geometry may
+ * be NULL: the eight geometry gradients are then not computed, and the volume-gradient
+ * adjoint needs no volume taps at all (autograd calls whose geometry needs no gradient).
+ * Replaces backward_cuda, csrc/src_cbct_forwardprojector_cuda.cu:634-729, including its
+ * compute_grad_volume switch (source_only_forward_projector.py:110); the geometry switch has
+ * no counterpart there (the reference always computes the eight sums). */
+size_t cbct_fp_backward_workspace_bytes_ex(int64_t D, int64_t H, int64_t W, int64_t A, int64_t U, int64_t V,
+                                           const cbct_fp_options* options);
+int cbct_fp_backward_ex(const float* grad_projection, const float* volume,
+                        int64_t D, int64_t H, int64_t W,
+                        const float* angles, const float* sid, const float* sod,
+                        const float* det_u0, const float* det_v0,
+                        const float* src_x, const float* src_y, const float* src_z,
+                        int64_t A, int64_t U, int64_t V,
+                        float pixel_size, float step_size, float voxel_size,
+                        int compute_grad_volume, float off_x, float off_y, float off_z,
+                        float* grad_volume, float* grad_geometry, const cbct_fp_options* options,
+                        void* workspace, size_t workspace_bytes, void* stream);
+
 /* Exact number of trilinear samples of the forward schedule (the sum over
  * rays of the loop trip counts of cbct_fp_forward), accumulated into
  * count[0] (a device uint64 the caller zeroes).  No reference counterpart:
@@ -109,8 +161,9 @@ int cbct_fp_count_samples(int64_t D, int64_t H, int64_t W,
  * for cbct_fp_backward with the volume gradient; cbct_fp_set_tuning sets both
  * and cbct_fp_get_tuning reads op 0.  Every setting gives the same projections
  * bit for bit; geometry gradients are bit-identical for a given layout
- * (another layout only changes their FP32 summation order).  Not synchronised
- * with calls in flight on other host threads. */
+ * (another layout only changes their FP32 summation order).  These are
+ * process-wide defaults (reads and writes are serialised); a caller that must
+ * not depend on them passes cbct_fp_options to the *_ex entry points. */
 int cbct_fp_set_tuning(int use_slab, int slab_thickness, int lds_floats, int layout);
 int cbct_fp_get_tuning(int* out4);
 int cbct_fp_set_tuning_op(int op, int use_slab, int slab_thickness, int lds_floats, int layout);
@@ -154,6 +207,21 @@ int cbct_bp_backward(const float* grad_volume, const float* projections,
                      float off_x, float off_y, float off_z,
                      float* grad_projections, float* grad_geometry,
                      void* workspace, size_t workspace_bytes, void* stream);
+
+/* cbct_bp_backward with per-call options (only options->deterministic is used; NULL: the
+ * process-wide default), whose workspace is cbct_bp_backward_workspace_bytes_ex.  Replaces the
+ * same backward_cuda, csrc/src_cbct_backprojector_cuda.cu:423-522. */
+size_t cbct_bp_backward_workspace_bytes_ex(int64_t A, int64_t U, int64_t V, int64_t D, int64_t H, int64_t W,
+                                           const cbct_fp_options* options);
+int cbct_bp_backward_ex(const float* grad_volume, const float* projections,
+                        int64_t A, int64_t U, int64_t V,
+                        const float* angles, const float* sid, const float* sod,
+                        const float* det_u0, const float* det_v0,
+                        const float* src_x, const float* src_y, const float* src_z,
+                        float pixel_size, int64_t D, int64_t H, int64_t W, float voxel_size,
+                        float off_x, float off_y, float off_z,
+                        float* grad_projections, float* grad_geometry, const cbct_fp_options* options,
+                        void* workspace, size_t workspace_bytes, void* stream);
 
 /* ---- FDK ---------------------------------------------------------------- */
 

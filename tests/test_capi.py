@@ -39,9 +39,13 @@ def test_version_and_workspace_queries():
     base = L.cbct_fp_backward_workspace_bytes(64, 64, 64, 10, 32, 32)
     assert base == L.cbct_fp_workspace_bytes(10, 32, 32)
     with _native.deterministic():
-        det = L.cbct_fp_backward_workspace_bytes(64, 64, 64, 10, 32, 32)
+        opts = _native.options(_native.TUNING_SCATTER)
+        det = L.cbct_fp_backward_workspace_bytes_ex(64, 64, 64, 10, 32, 32, ctypes.byref(opts))
     assert det == base + 64 ** 3 * 8 + 256
     assert not _native.get_deterministic()
+    # the thread's setting never reaches the process-wide default
+    assert L.cbct_fp_backward_workspace_bytes(64, 64, 64, 10, 32, 32) == base
+    assert L.cbct_fp_get_deterministic() == 0
     assert L.cbct_bp_workspace_bytes(360, 256, 256, 256) == 360 * 64
     assert L.cbct_fdk_workspace_bytes(360, 512, 384) >= 360 * 512 * 384 * 4
 

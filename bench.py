@@ -10,9 +10,17 @@ gradients) for a fixed upstream gradient, inputs resident in HBM.
 
 Prints ONE JSON line:
   value      GUPS = 2*D*H*W*A / (t_fwd + t_adj) / 1e9
-  roofline   algorithmic tap traffic of the dominant kernel (forward: 32 B per
-             trilinear sample + 4 B per ray; adjoint: 64 B per sample + 4 B per
-             ray + 4 B per voxel) over its HIP-event duration, against 8 TB/s
+  roofline   the roof the dominant kernel is measured against.  The projector kernels are
+             bound by VALU issue (the counters show it nearest its roof; HBM and LDS are
+             further from theirs): achieved = SQ_INSTS_VALU per launch (a constant of the
+             workload and kernel, from rocprofv3 --pmc, kept below) x 64 lanes / the
+             kernel's HIP-event duration measured here; peak = 2 wave instructions per
+             CU-cycle x 64 lanes x 256 CUs x 2.4 GHz = 78.6 TOP/s (the 157.3 TFLOP/s FP32
+             vector peak counted per lane-instruction).  Beside it: the measured HBM traffic
+             and its fraction of 8 TB/s, the LDS bank-conflict share, and the tap model's
+             bytes (forward: 32 B per trilinear sample + 4 B per ray; adjoint: 64 B per
+             sample + 4 B per ray + 4 B per voxel) over the duration, as a ratio to 8 TB/s
+             (above 1 because LDS serves most taps: it is not a roof).
   cpu_baseline  the FP32 CPU oracle (OpenMP) on a bounded subset of views.
 """
 
@@ -30,6 +38,9 @@ for _p in (ROOT, os.path.join(ROOT, "diff-geocali_amd")):
 
 METRIC = "fwd+adjoint GUPS at 256³×720 views; calib s to 0.1 px, learned vs plain"
 HBM_PEAK_GBPS = 8000.0
+# MI355X VALU issue peak in lane-instructions per second: 2 wave64 instructions per CU-cycle x 64
+# lanes x 256 CUs x 2.4 GHz (= the published 157.3 TFLOP/s FP32 vector peak / 2 FLOP per FMA).
+VALU_PEAK_TOPS = 2 * 64 * 256 * 2.4e9 / 1e12
 
 WORKLOADS = {
     "geometry-gradients-256c-720v": dict(D=256, A=720, U=768, V=640, pixel=0.6, sod=600.0, sid=1100.0,
@@ -49,16 +60,70 @@ WORKLOADS[BP_WORKLOAD] = dict(D=256, A=360, U=512, V=384, pixel=0.8, sod=500.0, 
 PREP_WORKLOAD = "preprocess-720v-768x640-u16"
 WORKLOADS[PREP_WORKLOAD] = dict(A=720, U=768, V=640)
 DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
-# Physical HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs of
-# `bench.py --pmc-probe` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB copy in the
-# same runs): adjoint fp_slab_kernel<2> at 256^3 x 720 views, 768x640 detector.
-MEASURED_TRAFFIC = {("geometry-gradients-256c-720v", "adjoint"): 103_239_016 * 1024 * 2.0 + 137_697_478 * 1024,
-                    ("geometry-gradients-256c-720v", "forward"): 91_308_190 * 1024 * 2.0 + 1_465_094 * 1024,
-                    # forward at 360 views, 512x384 (FDK round trip); backprojection backward at
-                    # 256^3 x 360 views (same counters, same factors)
-                    ("fdk-roundtrip-256c-360v", "forward"): 47_640_337 * 1024 * 2.0 + 296_104 * 1024,
-                    ("backprojection-256c-360v", "bp_backward"): 1_595_804 * 1024 * 2.0 + 6_993_152 * 1024,
-                    ("backprojection-256c-360v", "bp_forward"): 8_328_817 * 1024 * 2.0 + 172_207 * 1024}
+# Counters per launch of the projector kernels, from rocprofv3 --pmc runs of `bench.py --pmc-probe`
+# (one forward, one adjoint; summaries and commands in profiles/): SQ_INSTS_VALU (wave
+# instructions), SQ_LDS_BANK_CONFLICT (cycles, summed over CUs), GRBM_GUI_ACTIVE (cycles, summed
+# over the 8 XCDs), FETCH_SIZE and WRITE_SIZE (KiB).  A 1 GiB copy in the same runs reads 524,306
+# KiB of FETCH_SIZE and 1,048,576 KiB of WRITE_SIZE, so HBM bytes = FETCH_SIZE x 2.0 + WRITE_SIZE x
+# 1.0 (KiB x 1024).  The counts depend only on the workload, the kernel and its tuning.
+MEASURED = {
+    ("geometry-gradients-256c-720v", "adjoint"): dict(valu=111_882_127_715, lds_conflict=38_899_405_936,
+                                                      grbm=3_130_928_217, fetch_kib=103_288_638, write_kib=137_723_562),
+    ("geometry-gradients-256c-720v", "forward"): dict(valu=54_536_859_549, lds_conflict=9_670_306_160,
+                                                      grbm=1_271_345_956, fetch_kib=91_349_087, write_kib=1_466_077),
+    # forward at 360 views, 512x384 (FDK round trip); backprojection backward / forward at 256^3 x 360
+    # views (HBM counters only)
+    ("fdk-roundtrip-256c-360v", "forward"): dict(fetch_kib=47_640_337, write_kib=296_104),
+    ("backprojection-256c-360v", "bp_backward"): dict(fetch_kib=1_595_804, write_kib=6_993_152),
+    ("backprojection-256c-360v", "bp_forward"): dict(fetch_kib=8_328_817, write_kib=172_207),
+}
+CUS, XCDS = 256, 8
+
+
+def traffic_bytes(m):
+    """Measured HBM bytes per launch (None without the two counters)."""
+    if m is None or "fetch_kib" not in m:
+        return None
+    return m["fetch_kib"] * 1024 * 2.0 + m["write_kib"] * 1024 * 1.0
+
+
+def roofline(workload, kernel, seconds, tap_bytes, measured_ok=True, copy_gbps=None):
+    """The roofline object of one kernel of a workload (see the module docstring)."""
+    m = MEASURED.get((workload, kernel)) if measured_ok else None
+    traffic = traffic_bytes(m)
+    tap_gbps = tap_bytes / seconds / 1e9 if tap_bytes else None
+    out = {"kernel": kernel, "traffic": traffic,
+           "hbm_GBps": None if traffic is None else traffic / seconds / 1e9,
+           "hbm_frac": None if traffic is None else traffic / seconds / 1e9 / HBM_PEAK_GBPS,
+           "tap_model_GBps": tap_gbps, "tap_model_ratio": None if tap_gbps is None else tap_gbps / HBM_PEAK_GBPS,
+           "copy_GBps": copy_gbps}
+    if m is not None and "valu" in m:
+        achieved = m["valu"] * 64 / seconds / 1e12
+        out.update(bound="valu", achieved=achieved, peak=VALU_PEAK_TOPS, unit="TOP/s (VALU lane-instructions)",
+                   frac=achieved / VALU_PEAK_TOPS, valu_insts_per_launch=m["valu"],
+                   lds_bank_conflict_share=m["lds_conflict"] / CUS / (m["grbm"] / XCDS))
+    elif traffic is not None:
+        achieved = traffic / seconds / 1e9
+        out.update(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBPS, unit="GB/s", frac=achieved / HBM_PEAK_GBPS)
+    else:
+        out.update(bound=None, achieved=None, peak=None, unit=None, frac=None)
+    return out
+
+
+def cpu_identity():
+    """Model name of the host CPU (from /proc/cpuinfo) and the hardware threads visible."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
 PHANTOM_SEED = 20261203
 PERTURB_SEED = 7
 
@@ -162,7 +227,8 @@ def cpu_baseline(w, vol_np, geom_np, budget_s):
             break
     D = w["D"]
     gups = 2.0 * D * D * D * done / t_total / 1e9
-    return {"value": gups, "unit": "GUPS", "cores": threads, "kind": "port",
+    return {"value": gups, "unit": "GUPS", "cores": threads, "kind": "port", "cpu_model": cpu_identity()[0],
+            "threads_visible": cpu_identity()[1],
             "sample": f"{done} of {A} views (uniformly spaced), forward + adjoint with volume gradient, "
                       f"FP32 reference-schedule C oracle, {t_total:.1f} s"}
 
@@ -188,7 +254,8 @@ def cpu_baseline_fdk(w, vol_np, geom_np, budget_s):
         done += 1
         if t_total >= budget_s:
             break
-    return {"value": 2.0 * D ** 3 * done / t_total / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
+    return {"value": 2.0 * D ** 3 * done / t_total / 1e9, "unit": "GUPS", "cores": threads, "kind": "port", "cpu_model": cpu_identity()[0],
+            "threads_visible": cpu_identity()[1],
             "sample": f"{done} of {A} views (uniformly spaced): forward projection (FP32 C oracle), FDK filter "
                       f"(numpy FP64) and backprojection (FP32 C oracle), {t_total:.1f} s"}
 
@@ -212,7 +279,8 @@ def cpu_baseline_bp(w, geom_np, proj, gvol_np, budget_s):
         done += 1
         if t_total >= budget_s:
             break
-    return {"value": 2.0 * D ** 3 * done / t_total / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
+    return {"value": 2.0 * D ** 3 * done / t_total / 1e9, "unit": "GUPS", "cores": threads, "kind": "port", "cpu_model": cpu_identity()[0],
+            "threads_visible": cpu_identity()[1],
             "sample": f"{done} of {A} views (uniformly spaced): backprojection + backward, FP32 C oracle, "
                       f"{t_total:.1f} s"}
 
@@ -274,7 +342,6 @@ def calibration_bench(args, torch, device):
     rays = sum(s["A"] * s["U"] * s["V"] for s in specs)
     # fused march taps: 32 B per trilinear sample (the gradient reuses them) + 4 B measured per ray
     bytes_all = (32.0 * samples + 4.0 * rays) * marches
-    achieved = bytes_all / t / 1e9
     err0 = [calibration.reprojection_error_px(g, p.true_geometry, tuple(p.volume.shape), p.voxel_size,
                                               p.pixel_size) for g, p in zip(starts, probs)]
     err1 = [calibration.reprojection_error_px(p.geometry, p.true_geometry, tuple(p.volume.shape), p.voxel_size,
@@ -292,8 +359,9 @@ def calibration_bench(args, torch, device):
                    "reprojection_px_median_before": med(err0), "reprojection_px_median_after": med(err1),
                    "loss_ratio_median": med(loss_ratio),
                    "tunings": sorted({str(p.tuning) for p in probs})},
-        "roofline": {"bound": "hbm", "kernel": "fused calibration march (whole graph timed)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
+        # the whole graph is timed; one march is its time over the number of marches
+        "roofline": roofline(CALIB_WORKLOAD, "march", t / marches, bytes_all / marches,
+                             measured_ok=args.solver == "adam" and nsteps == w["adam_steps"]),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_calibration(probs, args.cpu_seconds)
@@ -326,7 +394,8 @@ def cpu_baseline_calibration(probs, budget_s):
         views += 1
         if t_total >= budget_s:
             break
-    return {"value": done_updates / t_total / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
+    return {"value": done_updates / t_total / 1e9, "unit": "GUPS", "cores": threads, "kind": "port", "cpu_model": cpu_identity()[0],
+            "threads_visible": cpu_identity()[1],
             "sample": f"one view of each of {views} problems: forward projection + geometry-only adjoint "
                       f"(FP32 C oracle), {t_total:.1f} s"}
 
@@ -495,16 +564,13 @@ def backprojection_bench(args, torch, device):
     N = D ** 3 * A
     b_bp, b_bb = 16.0 * N + 4.0 * D ** 3, 32.0 * N + 4.0 * D ** 3
     dom = ("bp_backward", b_bb, t_bb) if t_bb >= t_bp else ("bp_forward", b_bp, t_bp)
-    achieved = dom[1] / dom[2] / 1e9
     out = {
         "metric": METRIC, "value": 2.0 * N / (t_bp + t_bb) / 1e9, "unit": "GUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": BP_WORKLOAD, "volume": [D] * 3, "views": A, "detector": [U, V],
                    "t_bp_ms": t_bp * 1e3, "t_bp_backward_ms": t_bb * 1e3, "deterministic": args.deterministic},
-        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": None if args.deterministic else MEASURED_TRAFFIC.get((BP_WORKLOAD, dom[0]))},
+        "roofline": roofline(BP_WORKLOAD, dom[0], dom[2], dom[1], measured_ok=not args.deterministic),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_bp(w, geom_np, proj, gvol.cpu().numpy(), args.cpu_seconds)
@@ -543,6 +609,7 @@ def preprocess_bench(args, torch, device):
         "vs_baseline": None, "dtype": "u16->f32", "data": "synthetic",
         "config": {"workload": PREP_WORKLOAD, "views": A, "detector": [U, V], "layout_in": "AVU",
                    "ms_per_pass": t * 1e3},
+        # algorithmic bytes (6 per pixel): the kernel reads and writes each byte once
         "roofline": {"bound": "hbm", "kernel": "preprocess_kernel<uint16_t,1>", "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
     }), flush=True)
@@ -634,7 +701,6 @@ def main():
         b_adj = 64.0 * S + 4.0 * rays + 4.0 * D * D * D
         second = "adjoint"
     dominant = (second, b_adj, t_adj) if t_adj >= t_fwd else ("forward", b_fwd, t_fwd)
-    achieved = dominant[1] / dominant[2] / 1e9
     try:
         copy_bw = copy_rate_gbps(torch, device)
     except Exception:  # pragma: no cover
@@ -659,12 +725,12 @@ def main():
                    "fp_tuning": {"gather": list(_native.get_fp_tuning(_native.TUNING_GATHER)),
                                  "adjoint": list(_native.get_fp_tuning(_native.TUNING_SCATTER))},
                    "deterministic": args.deterministic},
-        "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": None if (args.fp_tuning or args.deterministic)
-                     else MEASURED_TRAFFIC.get((args.workload, dominant[0])),
-                     "pair_frac": (b_fwd + b_adj) / (t_fwd + t_adj) / 1e9 / HBM_PEAK_GBPS,
-                     "copy_GBps": copy_bw},
+        "roofline": roofline(args.workload, dominant[0], dominant[2], dominant[1],
+                             measured_ok=not (args.fp_tuning or args.deterministic), copy_gbps=copy_bw),
+        # the other kernel of the step, against the same roofs
+        "roofline_other": (roofline(args.workload, "forward", t_fwd, b_fwd,
+                                    measured_ok=not (args.fp_tuning or args.deterministic))
+                           if dominant[0] != "forward" else None),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = (cpu_baseline_fdk(w, vol_np, geom_np, args.cpu_seconds) if fdk

@@ -72,6 +72,7 @@ _SIGNATURES = {
     "cbct_calib_run": ([_P, _INT, _INT, _INT, _P, _F, _F, _F, _F, _INT, _P, _SZ, _P], _INT),
     "cbct_calib_lm_workspace_bytes": ([_P, _INT], _SZ),
     "cbct_calib_run_lm": ([_P, _INT, _INT, _F, _INT, _P, _SZ, _P], _INT),
+    "cbct_calib_release_graphs": ([], _INT),
     "cbct_preprocess": ([_P, _INT, _P, _P, _I64, _I64, _I64, _INT, _F, _INT, _P, _P], _INT),
     # learned warm start (include/cbct_nn.h)
     "cbct_nn_gemm_workspace_bytes": ([_I64, _I64, _INT], _SZ),

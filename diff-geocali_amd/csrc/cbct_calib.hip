@@ -18,6 +18,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 namespace cbct {
@@ -86,8 +89,7 @@ __global__ __launch_bounds__(256) void calib_reduce_kernel(const CalibDev* __res
         } else if (k < 8) {
             d.ggeo[k * d.A + a] = s;
         } else {
-            d.vloss[a] = s;
-            atomicAdd(d.loss_hist + step, s);
+            d.vloss[a] = s;          // summed per problem in a fixed order by calib_loss_kernel
         }
     }
 }
@@ -118,7 +120,7 @@ __global__ void calib_lm_kernel(const CalibDev* __restrict__ probs, int nprob, i
         lambda = fminf(st[8 + LM_NE] * 4.0f, 1e7f);
     }
     st[8 + LM_NE] = lambda;
-    atomicAdd(d.loss_hist + iter, st[8 + LM_NE + 1]);
+    d.vloss[a] = st[8 + LM_NE + 1];   // the accepted loss, summed by calib_loss_kernel
     if (!update) {
         for (int k = 0; k < 8; ++k) d.geom[k * A + a] = st[k];      // finish at the accepted geometry
         return;
@@ -167,6 +169,36 @@ __global__ void calib_lm_kernel(const CalibDev* __restrict__ probs, int nprob, i
         }
     }
     for (int k = 0; k < 8; ++k) d.geom[k * A + a] = (float)((double)st[k] + x[k]);
+}
+
+// loss_hist[step] of every problem: the fixed-order sum of its per-view losses (one workgroup per
+// problem), so the history is bitwise reproducible and equal whatever the batch around it.
+__global__ __launch_bounds__(256) void calib_loss_kernel(const CalibDev* __restrict__ probs, int step) {
+    const CalibDev& d = probs[blockIdx.x];
+    float acc = 0.f;
+    for (int a = threadIdx.x; a < d.A; a += 256) acc += d.vloss[a];
+    __shared__ float sm[4];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) d.loss_hist[step] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+}
+
+// Copies up to DESC_BATCH problem descriptors, passed by value, into device memory: the upload
+// needs no host buffer that outlives the call and no synchronisation.
+constexpr int DESC_BATCH = 8;
+struct DescBatch {
+    CalibDev d[DESC_BATCH];
+};
+static_assert(sizeof(DescBatch) <= 3072, "descriptor batch must fit the kernel argument space");
+
+__global__ void calib_desc_kernel(DescBatch batch, int n, CalibDev* __restrict__ out) {
+    const int words = (int)(sizeof(CalibDev) / sizeof(uint32_t));
+    static_assert(sizeof(CalibDev) % sizeof(uint32_t) == 0, "descriptor of whole words");
+    for (int i = threadIdx.x; i < n * words; i += blockDim.x) {
+        const int k = i / words, w = i % words;
+        reinterpret_cast<uint32_t*>(out + k)[w] = reinterpret_cast<const uint32_t*>(&batch.d[k])[w];
+    }
 }
 
 struct AdamArgs {
@@ -218,7 +250,49 @@ static CalibLayout calib_layout(const cbct_calib_problem* pr, int n, int solver)
     return L;
 }
 
-int launch_fill(float* p, int64_t n, float value, hipStream_t stream);
+// ---- instantiated-graph cache of cbct_calib_run / cbct_calib_run_lm
+struct GraphKey {
+    int device;
+    const void* ws;
+    int nproblems, nsteps, first_step, solver, total_tiles, total_views, total_params, max_cap, grid;
+    float lr[8];
+    float lr_decay, beta1, beta2, eps, lambda0;
+};
+
+constexpr size_t GRAPH_CACHE_ENTRIES = 8;
+static std::mutex g_graph_mutex;
+static std::vector<std::pair<GraphKey, hipGraphExec_t>> g_graphs;   // oldest first
+
+static bool same_key(const GraphKey& a, const GraphKey& b) { return std::memcmp(&a, &b, sizeof(GraphKey)) == 0; }
+
+static hipGraphExec_t graph_cache_find(const GraphKey& k) {
+    std::lock_guard<std::mutex> lock(g_graph_mutex);
+    for (auto& e : g_graphs)
+        if (same_key(e.first, k)) return e.second;
+    return nullptr;
+}
+
+// An evicted graph that is still running is freed when it completes (hipGraphExecDestroy).
+static void graph_cache_insert(const GraphKey& k, hipGraphExec_t exec) {
+    std::lock_guard<std::mutex> lock(g_graph_mutex);
+    if (g_graphs.size() >= GRAPH_CACHE_ENTRIES) {
+        (void)hipGraphExecDestroy(g_graphs.front().second);
+        g_graphs.erase(g_graphs.begin());
+    }
+    g_graphs.emplace_back(k, exec);
+}
+
+static int graph_cache_clear() {
+    std::lock_guard<std::mutex> lock(g_graph_mutex);
+    hipError_t first = hipSuccess;
+    for (auto& e : g_graphs) {
+        const hipError_t r = hipGraphExecDestroy(e.second);
+        if (first == hipSuccess) first = r;
+    }
+    g_graphs.clear();
+    if (first != hipSuccess) return hip_status(first, "graph release");
+    return CBCT_OK;
+}
 
 }  // namespace cbct
 
@@ -328,19 +402,19 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
     }
     CalibDev* ddev = (CalibDev*)(ws + L.descs);
     unsigned* counter = (unsigned*)(ws + L.counter);
-    // descriptors: a one-time upload, ordered on the stream before every step
-    CBCT_HIP(hipMemcpyAsync(ddev, host.data(), sizeof(CalibDev) * nproblems, hipMemcpyHostToDevice, stream),
-             "descriptor upload");
-    CBCT_HIP(hipStreamSynchronize(stream), "descriptor upload");
-    for (int i = 0; i < nproblems; ++i) {
-        int st;
-        if ((st = launch_fill(pr[i].loss_history, nsteps + 1, 0.0f, stream))) return st;
+    // descriptors: written by kernels whose arguments carry them, ordered on the stream before
+    // every step (no host synchronisation)
+    for (int i0 = 0; i0 < nproblems; i0 += DESC_BATCH) {
+        DescBatch batch;
+        const int n = std::min(DESC_BATCH, nproblems - i0);
+        for (int k = 0; k < n; ++k) batch.d[k] = host[i0 + k];
+        calib_desc_kernel<<<1, 256, 0, stream>>>(batch, n, ddev + i0);
+        CBCT_LAUNCHED("calibration descriptor upload");
     }
-    int device = 0, cus = 0, per_cu = 0;
+    int device = 0, cus = 0;
     CBCT_HIP(hipGetDevice(&device), "hipGetDevice");
     CBCT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
     const int grid = std::max(1, std::min(total_tiles, cus * 8));
-    (void)per_cu;
     AdamArgs h;
     for (int k = 0; k < 8; ++k) h.lr[k] = lr8 ? lr8[k] : 0.f;
     h.beta1 = beta1; h.beta2 = beta2; h.eps = eps;
@@ -358,10 +432,14 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
             calib_lm_kernel<<<(unsigned)ceil_div(total_views, 64), 64, 0, st_>>>(ddev, nproblems, total_views, step,
                                                                                  lambda0, update ? 1 : 0);
             CBCT_LAUNCHED("calibration LM kernel launch");
+            calib_loss_kernel<<<(unsigned)nproblems, 256, 0, st_>>>(ddev, step);
+            CBCT_LAUNCHED("calibration loss kernel launch");
             return CBCT_OK;
         }
         calib_reduce_kernel<9><<<(unsigned)total_views, 256, 0, st_>>>(ddev, nproblems, step);
         CBCT_LAUNCHED("calibration reduce kernel launch");
+        calib_loss_kernel<<<(unsigned)nproblems, 256, 0, st_>>>(ddev, step);
+        CBCT_LAUNCHED("calibration loss kernel launch");
         if (update) {
             const int t = first_step + step;
             const double decay = std::pow((double)lr_decay, t);
@@ -375,13 +453,29 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
         return CBCT_OK;
     };
 
-    if (use_graph) {
-        // Capture on a private stream (the caller's may be the null stream, which cannot be
-        // captured), then launch the graph on the caller's stream.
+    if (!use_graph) {
+        for (int s = 0; s < nsteps; ++s) {
+            int st;
+            if ((st = enqueue(s, true, stream))) return st;
+        }
+        return enqueue(nsteps, false, stream);
+    }
+    // All steps in one graph.  Its kernels read every per-problem pointer from the descriptors in
+    // the workspace (rewritten above on every call), so an instantiated graph depends only on the
+    // key below and is kept for later calls with the same key (cbct_calib_release_graphs frees them).
+    GraphKey key;
+    std::memset(&key, 0, sizeof(key));    // padding included, the key is compared bytewise
+    key.device = device; key.ws = ws; key.nproblems = nproblems; key.nsteps = nsteps; key.first_step = first_step;
+    key.solver = solver; key.total_tiles = total_tiles; key.total_views = total_views;
+    key.total_params = total_params; key.max_cap = max_cap; key.grid = grid;
+    for (int k = 0; k < 8; ++k) key.lr[k] = lr8 ? lr8[k] : 0.f;
+    key.lr_decay = lr_decay; key.beta1 = beta1; key.beta2 = beta2; key.eps = eps; key.lambda0 = lambda0;
+    hipGraphExec_t exec = graph_cache_find(key);
+    if (exec == nullptr) {
+        // capture on a private stream (the caller's may be the null stream, which cannot be captured)
         hipStream_t cs = nullptr;
         CBCT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
         hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
         hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) { (void)hipStreamDestroy(cs); return hip_status(e, "graph capture begin"); }
         int st = CBCT_OK;
@@ -392,18 +486,14 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
         if (st != CBCT_OK) { if (graph) (void)hipGraphDestroy(graph); return st; }
         if (e != hipSuccess) return hip_status(e, "graph capture end");
         e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (e != hipSuccess) { (void)hipGraphDestroy(graph); return hip_status(e, "graph instantiate"); }
-        e = hipGraphLaunch(exec, stream);
-        hipError_t e2 = hipStreamSynchronize(stream);
-        (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) return hip_status(e, "graph launch");
-        if (e2 != hipSuccess) return hip_status(e2, "graph execution");
-        return CBCT_OK;
+        if (e != hipSuccess) return hip_status(e, "graph instantiate");
+        graph_cache_insert(key, exec);
     }
-    for (int s = 0; s < nsteps; ++s) {
-        int st;
-        if ((st = enqueue(s, true, stream))) return st;
-    }
-    return enqueue(nsteps, false, stream);
+    CBCT_HIP(hipGraphLaunch(exec, stream), "graph launch");
+    return CBCT_OK;
+}
+
+extern "C" int cbct_calib_release_graphs(void) {
+    return graph_cache_clear();
 }

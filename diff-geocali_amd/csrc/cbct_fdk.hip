@@ -159,10 +159,13 @@ __global__ __launch_bounds__(256) void fdk_filter_kernel(
 //   w = sin^2(pi/4 * beta / (delta - gamma))              for 0 <= beta < 2 delta - 2 gamma
 //     = 1                                                  up to pi - 2 gamma
 //     = sin^2(pi/4 * (pi + 2 delta - beta) / (delta + gamma)) up to pi + 2 delta
-// makes every line's weights sum to one.
+// makes every line's weights sum to one.  view_scale (reconstruction): the weights also carry
+// the ratio of the short scan's view weight, the mean angular step (angle_{A-1} - angle_0)/(A-1),
+// to the pi/A that the backprojection applies, so the angular range never has to be read back
+// to the host.
 __global__ void parker_weights_kernel(const float* __restrict__ angles, int A, int U,
                                       const float* __restrict__ u0a, float pix,
-                                      const float* __restrict__ sid, float* __restrict__ w) {
+                                      const float* __restrict__ sid, int view_scale, float* __restrict__ w) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)A * U) return;
     const int a = (int)(i / U), u = (int)(i % U);
@@ -180,6 +183,10 @@ __global__ void parker_weights_kernel(const float* __restrict__ angles, int A, i
     } else if (beta > pi - 2.f * g && beta <= pi + 2.f * delta) {
         const float t = sinf(pi / 4.f * (pi + 2.f * delta - beta) / (delta + g));
         val = t * t;
+    }
+    if (view_scale) {
+        const double step = ((double)angles[A - 1] - (double)angles[0]) / (double)(A - 1);
+        val = (float)((double)val * step * ((double)A / 3.14159265358979323846));
     }
     w[i] = val;
 }
@@ -204,7 +211,7 @@ static FdkLayout fdk_layout(int64_t A, int64_t U, int64_t V) {
 static int launch_filter(const float* proj, int64_t A, int64_t U, int64_t V, float pix,
                          const float* sod, const float* sid, float alpha, const float* u0a, const float* v0a,
                          const float* angles, float* out, float* rh, float* taps, float* pw,
-                         hipStream_t stream) {
+                         hipStream_t stream, int view_scale = 0) {
     const int pad = (int)fdk_pad(U);
     ramp_response_kernel<<<(pad + 255) / 256, 256, 0, stream>>>(pad, pix, alpha, rh);
     CBCT_LAUNCHED("ramp response kernel launch");
@@ -213,7 +220,8 @@ static int launch_filter(const float* proj, int64_t A, int64_t U, int64_t V, flo
     CBCT_LAUNCHED("filter taps kernel launch");
     if (angles) {
         parker_weights_kernel<<<(unsigned)ceil_div(A * U, (int64_t)256), 256, 0, stream>>>(angles, (int)A, (int)U,
-                                                                                          u0a, pix, sid, pw);
+                                                                                          u0a, pix, sid, view_scale,
+                                                                                          pw);
         CBCT_LAUNCHED("Parker weights kernel launch");
     }
     const int64_t blocks = A * ceil_div(U, FT_U) * ceil_div(V, FT_V);
@@ -303,24 +311,16 @@ extern "C" int cbct_fdk_reconstruct_ex(const float* projections, int64_t A, int6
     char* ws = (char*)workspace;
     ViewFrame* frames = (ViewFrame*)(ws + L.frames);
     float* filtered = (float*)(ws + L.filtered);
-    double scale = 3.14159265358979323846 / (double)A;       // full scan: (1/2) * 2 pi / A
-    if (short_scan) {
-        // Parker-weighted short scan: weight = the mean angular step (reads the angular range)
-        float ends[2];
-        CBCT_HIP(hipMemcpyAsync(&ends[0], angles, sizeof(float), hipMemcpyDeviceToHost, stream), "angle read");
-        CBCT_HIP(hipMemcpyAsync(&ends[1], angles + (A - 1), sizeof(float), hipMemcpyDeviceToHost, stream),
-                 "angle read");
-        CBCT_HIP(hipStreamSynchronize(stream), "angle read");
-        const double range = (double)ends[1] - (double)ends[0];
-        CBCT_CHECK_ARG(range > 3.14159265358979323846,
-                       "a short-scan reconstruction needs increasing angles spanning more than pi");
-        scale = range / (double)(A - 1);
-    }
+    // full scan: each view weighs (1/2) * 2 pi / A; a Parker short scan weighs the mean angular
+    // step, which the Parker weights carry as a factor (parker_weights_kernel), so the range is
+    // never read back to the host
+    const double scale = 3.14159265358979323846 / (double)A;
     const GeomPtrs gp{angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
     if ((st = launch_frames(gp, A, frames, stream))) return st;
     if ((st = launch_filter(projections, A, U, V, pixel_size, sod, sid, alpha, offset_preweight ? det_u0 : nullptr,
                             offset_preweight ? det_v0 : nullptr, short_scan ? angles : nullptr, filtered,
-                            (float*)(ws + L.rh), (float*)(ws + L.taps), (float*)(ws + L.parker), stream)))
+                            (float*)(ws + L.rh), (float*)(ws + L.taps), (float*)(ws + L.parker), stream,
+                            short_scan ? 1 : 0)))
         return st;
     const int chunk = (int)std::min<int64_t>(angle_chunk_size, A);
     return launch_bp_forward_fdk(filtered, frames, A, U, V, D, H, W, pixel_size, voxel_size,

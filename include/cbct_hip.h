@@ -255,7 +255,8 @@ int cbct_fdk_reconstruct(const float* projections, int64_t A, int64_t U, int64_t
  * short-scan weights (angles increasing, angles[A-1] - angles[0] > pi; delta =
  * (range - pi)/2).  cbct_fdk_reconstruct_ex: short_scan applies Parker weights
  * and weights each view by (angles[A-1] - angles[0]) / (A - 1) instead of pi/A
- * (it reads the two end angles and synchronises the stream once);
+ * (computed on the device; the angles must increase and span more than pi, which the
+ * library does not read back to check);
  * offset_preweight uses det_u0/det_v0 in the preweight. */
 int cbct_fdk_filter_ex(const float* projections, int64_t A, int64_t U, int64_t V,
                        float pixel_size, const float* sod, const float* sid, float alpha,
@@ -302,9 +303,11 @@ size_t cbct_calib_workspace_bytes(const cbct_calib_problem* problems, int nprobl
  * parameter group in the geometry order; step t (counted from first_step, so a
  * fit can be continued by a later call with the same Adam moments) uses
  * lr8 * lr_decay^t and the bias corrections 1 - beta^(t+1).  loss_history[i]
- * is the loss before the i-th update of this call.  use_graph != 0 captures all
- * steps in one hipGraph and launches it once.  The descriptor upload and, with
- * a graph, the end of the run synchronise the stream. */
+ * is the loss before the i-th update of this call (each a fixed-order sum of per-view losses,
+ * so bitwise reproducible).  use_graph != 0 captures all steps in one hipGraph and launches
+ * it; the instantiated graph is cached by the call's sizes, workspace address and optimiser
+ * settings (up to 8 graphs; cbct_calib_release_graphs frees them), so repeated calls with the
+ * same key launch it again without capturing.  No part of the call synchronises the stream. */
 int cbct_calib_run(const cbct_calib_problem* problems, int nproblems, int nsteps, int first_step,
                    const float* lr8, float lr_decay, float beta1, float beta2, float eps, int use_graph,
                    void* workspace, size_t workspace_bytes, void* stream);
@@ -318,6 +321,10 @@ int cbct_calib_run(const cbct_calib_problem* problems, int nproblems, int nsteps
  * adam_v may be NULL.  loss_history[i] = the accepted loss after i iterations;
  * the geometry ends at the accepted iterate.  No counterpart in the reference
  * (its calibration loop is left to user code). */
+/* Frees the instantiated graphs cached by cbct_calib_run / cbct_calib_run_lm (graphs still
+ * running are freed when they complete).  No reference counterpart. */
+int cbct_calib_release_graphs(void);
+
 size_t cbct_calib_lm_workspace_bytes(const cbct_calib_problem* problems, int nproblems);
 int cbct_calib_run_lm(const cbct_calib_problem* problems, int nproblems, int nsteps, float lambda0,
                       int use_graph, void* workspace, size_t workspace_bytes, void* stream);

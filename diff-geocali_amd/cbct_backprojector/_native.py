@@ -52,6 +52,8 @@ _SIGNATURES = {
     "cbct_fp_backward_workspace_bytes_ex": ([_I64] * 6 + [_P], _SZ),
     "cbct_fp_backward_ex": ([_P, _P, _I64, _I64, _I64] + [_P] * 8 + [_I64, _I64, _I64, _F, _F, _F, _INT,
                             _F, _F, _F, _P, _P, _P, _P, _SZ, _P], _INT),
+    "cbct_fp_backward_noalloc": ([_P, _P, _I64, _I64, _I64] + [_P] * 8 + [_I64, _I64, _I64, _F, _F, _F, _INT,
+                                 _F, _F, _F, _P, _P, _P, _P], _INT),
     "cbct_bp_backward_workspace_bytes_ex": ([_I64] * 6 + [_P], _SZ),
     "cbct_bp_backward_ex": ([_P, _P, _I64, _I64, _I64] + [_P] * 8 + [_F, _I64, _I64, _I64, _F, _F, _F, _F,
                             _P, _P, _P, _P, _SZ, _P], _INT),
@@ -323,6 +325,29 @@ def fp_backward(grad, volume, geometry, pixel_size, step_size, voxel_size, compu
                                      _ptr(gvol) if compute_grad_volume else None, _ptr(ggeo), ctypes.byref(o),
                                      _ptr(ws), ws_bytes, _stream(dev)))
     return gvol, ggeo
+
+
+# cbct_fp_backward_noalloc's fixed device scratch (CBCT_NOALLOC_SCRATCH_BYTES in cbct_hip.h)
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
+def fp_backward_noalloc(grad, volume, geometry, pixel_size, step_size, voxel_size, compute_grad_volume,
+                        off_x, off_y, off_z, grad_volume=None, grad_geometry=None, opts=None):
+    """fp_backward through the allocation-free entry point: the library allocates nothing (its
+    workspace is the fixed device scratch); the caller passes the outputs (``grad_volume``
+    [D,H,W] when the volume gradient is wanted, ``grad_geometry`` [8, A] or None)."""
+    _require_gpu(volume, "volume")
+    L = lib()
+    D, H, W = volume.shape
+    A, U, V = grad.shape
+    dev = volume.device
+    o = opts if opts is not None else options(TUNING_SCATTER if compute_grad_volume else TUNING_GATHER)
+    with torch.cuda.device(dev):
+        _check(L.cbct_fp_backward_noalloc(_ptr(grad), _ptr(volume), D, H, W, *[_ptr(g) for g in geometry],
+                                          A, U, V, pixel_size, step_size, voxel_size,
+                                          1 if compute_grad_volume else 0, off_x, off_y, off_z,
+                                          _ptr(grad_volume), _ptr(grad_geometry), ctypes.byref(o), _stream(dev)))
+    return grad_volume, grad_geometry
 
 
 def fp_count_samples(volume_shape, geometry, pixel_size, U, V, step_size, voxel_size,

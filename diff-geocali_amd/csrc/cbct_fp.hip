@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <string>
 
 namespace cbct {
 
@@ -572,6 +573,62 @@ extern "C" int cbct_fp_backward_ex(const float* grad_projection, const float* vo
     return fp_backward_impl(grad_projection, volume, D, H, W, g, A, U, V, pixel_size, step_size, voxel_size,
                             compute_grad_volume, off_x, off_y, off_z, grad_volume, grad_geometry, true, options,
                             workspace, workspace_bytes, stream_);
+}
+
+// ---- allocation-free adjoint
+// The workspace of cbct_fp_backward_noalloc (view frames, per-tile geometry partials and, in
+// deterministic mode, the int64 accumulator) lives in this fixed device array: each GPU the
+// library runs on holds its own copy of it until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the symbol): the device address
+// is looked up once per device with hipGetSymbolAddress and kept in g_scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[CBCT_NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The current device's address of the scratch array (looked up on the first call on the device).
+static int scratch_address(void** out) {
+    int dev = -1;
+    CBCT_HIP(hipGetDevice(&dev), "hipGetDevice");
+    CBCT_CHECK_ARG(dev >= 0 && dev < MAX_DEVICES, "device ordinal beyond the library's 64-device table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        CBCT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch symbol address");
+        CBCT_CHECK_ARG(p != nullptr, "scratch symbol address is null");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return CBCT_OK;
+}
+
+extern "C" int cbct_fp_backward_noalloc(const float* grad_projection, const float* volume,
+                                        int64_t D, int64_t H, int64_t W,
+                                        const float* angles, const float* sid, const float* sod,
+                                        const float* det_u0, const float* det_v0,
+                                        const float* src_x, const float* src_y, const float* src_z,
+                                        int64_t A, int64_t U, int64_t V,
+                                        float pixel_size, float step_size, float voxel_size,
+                                        int compute_grad_volume, float off_x, float off_y, float off_z,
+                                        float* grad_volume, float* grad_geometry, const cbct_fp_options* options,
+                                        void* stream_) {
+    int st;
+    if ((st = check_fp_common(D, H, W, A, U, V, pixel_size, step_size, voxel_size, off_x, off_y, off_z))) return st;
+    FpCall call;
+    if ((st = call_settings(options, compute_grad_volume ? 1 : 0, call))) return st;
+    const size_t need = cbct_fp_backward_workspace_bytes_ex(D, H, W, A, U, V, options);
+    if (need > (size_t)CBCT_NOALLOC_SCRATCH_BYTES)
+        return fail(CBCT_EWORKSPACE, "cbct_fp_backward_noalloc needs " + std::to_string(need) +
+                                         " bytes of scratch; its fixed scratch holds " +
+                                         std::to_string((size_t)CBCT_NOALLOC_SCRATCH_BYTES) +
+                                         " (use cbct_fp_backward_ex with a workspace)");
+    void* ws = nullptr;
+    if ((st = scratch_address(&ws))) return st;
+    const float* g[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
+    return fp_backward_impl(grad_projection, volume, D, H, W, g, A, U, V, pixel_size, step_size, voxel_size,
+                            compute_grad_volume, off_x, off_y, off_z, grad_volume, grad_geometry, true, options, ws,
+                            (size_t)CBCT_NOALLOC_SCRATCH_BYTES, stream_);
 }
 
 extern "C" int cbct_fp_count_samples(int64_t D, int64_t H, int64_t W,

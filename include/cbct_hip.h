@@ -138,6 +138,27 @@ int cbct_fp_backward_ex(const float* grad_projection, const float* volume,
                         float* grad_volume, float* grad_geometry, const cbct_fp_options* options,
                         void* workspace, size_t workspace_bytes, void* stream);
 
+/* Allocation-free cbct_fp_backward_ex.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving or calibration
+ * loops, harnesses that forbid allocation).  It takes no workspace: the workspace the call needs
+ * (cbct_fp_backward_workspace_bytes_ex with the same options) must fit the library's fixed
+ * device scratch of CBCT_NOALLOC_SCRATCH_BYTES (64 MiB; each GPU the library runs on holds one
+ * until the process ends), otherwise the call returns CBCT_EWORKSPACE without touching the
+ * device.  It calls no allocating routine.  All calls on one GPU share that GPU's scratch, so
+ * calls that overlap in time on one GPU (other streams, other host threads) race: order them.
+ * Every byte a call reads from the scratch is written by the same call first. */
+#define CBCT_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int cbct_fp_backward_noalloc(const float* grad_projection, const float* volume,
+                             int64_t D, int64_t H, int64_t W,
+                             const float* angles, const float* sid, const float* sod,
+                             const float* det_u0, const float* det_v0,
+                             const float* src_x, const float* src_y, const float* src_z,
+                             int64_t A, int64_t U, int64_t V,
+                             float pixel_size, float step_size, float voxel_size,
+                             int compute_grad_volume, float off_x, float off_y, float off_z,
+                             float* grad_volume, float* grad_geometry, const cbct_fp_options* options,
+                             void* stream);
+
 /* Exact number of trilinear samples of the forward schedule (the sum over
  * rays of the loop trip counts of cbct_fp_forward), accumulated into
  * count[0] (a device uint64 the caller zeroes).  No reference counterpart:

@@ -254,7 +254,7 @@ static CalibLayout calib_layout(const cbct_calib_problem* pr, int n, int solver)
 struct GraphKey {
     int device;
     const void* ws;
-    int nproblems, nsteps, first_step, solver, total_tiles, total_views, total_params, max_cap, grid;
+    int nproblems, nsteps, first_step, solver, total_tiles, total_views, total_params, max_cap, grid, big;
     float lr[8];
     float lr_decay, beta1, beta2, eps, lambda0;
 };
@@ -346,12 +346,14 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
     cbct_fp_get_tuning(dflt);
     std::vector<CalibDev> host(nproblems);
     int total_tiles = 0, total_views = 0, total_params = 0, max_cap = 256;
+    bool big = false;
     for (int i = 0; i < nproblems; ++i) {
         const cbct_calib_problem& q = pr[i];
         CBCT_CHECK_ARG(q.volume && q.measured && q.geometry && q.loss_history && (solver == 1 || (q.adam_m && q.adam_v)),
                        "problem pointers must be non-null");
-        CBCT_CHECK_ARG(q.D > 0 && q.H > 0 && q.W > 0 && q.D * q.H * q.W <= INT32_MAX,
-                       "volume dimensions must be positive and within the 2^31-1 element limit");
+        CBCT_CHECK_ARG(q.D > 0 && q.H > 0 && q.W > 0 && q.D < (1 << 24) && q.H < (1 << 24) && q.W < (1 << 24) &&
+                           q.H * q.W <= INT32_MAX && q.D * q.H <= INT32_MAX,
+                       "volume dimensions must be positive, below 2^24 and with planes within 2^31-1 elements");
         CBCT_CHECK_ARG(q.A > 0 && q.U > 0 && q.V > 0 && q.A * tiles_of(q.U, q.V) <= INT32_MAX,
                        "projection dimensions must be positive");
         int st;
@@ -399,6 +401,7 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
         total_tiles += s.nblocks;
         total_params += 8 * (int)q.A;
         max_cap = std::max(max_cap, s.cap);
+        big = big || q.D * q.H * q.W > INT32_MAX;
     }
     CalibDev* ddev = (CalibDev*)(ws + L.descs);
     unsigned* counter = (unsigned*)(ws + L.counter);
@@ -425,7 +428,7 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
         calib_frames_kernel<<<(unsigned)ceil_div(total_views, 128), 128, 0, st_>>>(ddev, nproblems, total_views);
         CBCT_LAUNCHED("calibration frames kernel launch");
         int st;
-        if ((st = launch_fp_slab_batched(ddev, nproblems, total_tiles, max_cap, solver, counter, grid, st_))) return st;
+        if ((st = launch_fp_slab_batched(ddev, nproblems, total_tiles, max_cap, solver, counter, grid, st_, big))) return st;
         if (solver == 1) {
             calib_reduce_kernel<LM_NE><<<(unsigned)total_views, 256, 0, st_>>>(ddev, nproblems, step);
             CBCT_LAUNCHED("calibration reduce kernel launch");
@@ -467,7 +470,7 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
     std::memset(&key, 0, sizeof(key));    // padding included, the key is compared bytewise
     key.device = device; key.ws = ws; key.nproblems = nproblems; key.nsteps = nsteps; key.first_step = first_step;
     key.solver = solver; key.total_tiles = total_tiles; key.total_views = total_views;
-    key.total_params = total_params; key.max_cap = max_cap; key.grid = grid;
+    key.total_params = total_params; key.max_cap = max_cap; key.grid = grid; key.big = big ? 1 : 0;
     for (int k = 0; k < 8; ++k) key.lr[k] = lr8 ? lr8[k] : 0.f;
     key.lr_decay = lr_decay; key.beta1 = beta1; key.beta2 = beta2; key.eps = eps; key.lambda0 = lambda0;
     hipGraphExec_t exec = graph_cache_find(key);

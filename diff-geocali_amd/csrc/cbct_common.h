@@ -236,7 +236,8 @@ __device__ __forceinline__ float frac(float x) { return __builtin_amdgcn_fractf(
 // Trilinear corner set of a sample: clamp-to-edge corner offsets and raw
 // fractions (csrc/src_cbct_forwardprojector_cuda.cu:186-229).
 struct Corners {
-    int i000, dx1, dy1, dz1;   // base offset and clamped +1 steps per axis
+    int64_t i000;              // base offset (64-bit: a volume may exceed 2^31 voxels)
+    int dx1, dy1, dz1;         // clamped +1 steps per axis (a plane H*W stays below 2^31)
     float fx, fy, fz;
 };
 
@@ -248,7 +249,7 @@ __device__ __forceinline__ Corners corners(const Box& b, float x, float y, float
     const int y0 = max(0, min(yr, b.H - 1)), y1 = max(0, min(yr + 1, b.H - 1));
     const int z0 = max(0, min(zr, b.D - 1)), z1 = max(0, min(zr + 1, b.D - 1));
     const int P = b.H * b.W;
-    c.i000 = z0 * P + y0 * b.W + x0;
+    c.i000 = (int64_t)z0 * P + (y0 * b.W + x0);
     c.dx1 = x1 - x0;
     c.dy1 = (y1 - y0) * b.W;
     c.dz1 = (z1 - z0) * P;
@@ -278,7 +279,7 @@ __device__ __forceinline__ Cell cell_of(const Box& b, float x, float y, float z)
 struct Taps { float v000, v100, v010, v110, v001, v101, v011, v111; };
 
 // Eight taps at base offset i000 with strides (1, ry, rz) scaled by the 0/1 steps.
-__device__ __forceinline__ Taps load_taps_strided(const float* __restrict__ base, int i000,
+__device__ __forceinline__ Taps load_taps_strided(const float* __restrict__ base, int64_t i000,
                                                   int dx, int dy, int dz) {
     const float* p = base + i000;
     Taps t;

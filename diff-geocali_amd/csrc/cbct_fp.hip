@@ -347,7 +347,11 @@ static int check_fp_common(int64_t D, int64_t H, int64_t W, int64_t A, int64_t U
     CBCT_CHECK_ARG(D > 0 && H > 0 && W > 0, "every volume dimension must be positive");
     CBCT_CHECK_ARG(U > 0 && V > 0, "U and V must be positive");
     CBCT_CHECK_ARG(A > 0, "angles must contain at least one view");
-    CBCT_CHECK_ARG(D * H * W <= INT32_MAX, "volume exceeds the 2^31-1 element limit");
+    // voxel offsets are 64-bit; the per-axis limits keep a plane (H*W) and a column of planes
+    // (D*H) within 32 bits, and every voxel coordinate exact in FP32
+    CBCT_CHECK_ARG(D < (1 << 24) && H < (1 << 24) && W < (1 << 24),
+                   "every volume dimension must be below 2^24");
+    CBCT_CHECK_ARG(H * W <= INT32_MAX && D * H <= INT32_MAX, "volume planes exceed the 2^31-1 element limit");
     CBCT_CHECK_ARG(A * fp_tiles(U, V) <= INT32_MAX, "too many rays for one launch");
     int st;
     if ((st = check_positive_finite(pix, "pixel_size"))) return st;

@@ -56,7 +56,8 @@ int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream,
 // Mode 3 (fused forward + squared residual + geometry adjoint; solver 0) or mode 4 (normal
 // equations; solver 1) over every tile of every problem, persistent workgroups fed by the work
 // counter (which must be zero at launch).  Every problem of a launch uses the same solver.
+// big: some problem's volume has more than 2^31 - 1 voxels (64-bit voxel offsets).
 int launch_fp_slab_batched(const CalibDev* probs, int nprob, int total_tiles, int max_cap, int solver,
-                           unsigned* counter, int grid, hipStream_t stream);
+                           unsigned* counter, int grid, hipStream_t stream, bool big);
 
 }  // namespace cbct

@@ -45,8 +45,11 @@ __device__ __forceinline__ void pixel_dir(const ViewFrame& f, float u, float v, 
 // GEO (MODE 2 only): with false, the geometry gradients are not wanted -- the adjoint then needs
 // no volume taps at all (the scatter depends only on the sample positions and the upstream
 // weights): no staged volume box, no interpolation, no geometry sums, one LDS box per workgroup.
-template <int MODE, bool DIV, bool ZOFF = false, int CAPC = 0, bool GEO = true>
+// BIG: the volume has more than 2^31 - 1 voxels, so voxel offsets are formed in 64 bits (the
+// 32-bit forms of the other variants are cheaper in the staging and flush loops).
+template <int MODE, bool DIV, bool ZOFF = false, int CAPC = 0, bool GEO = true, bool BIG = false>
 __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
+    using VoxelIndex = typename std::conditional<BIG, int64_t, int>::type;
     static_assert(GEO || MODE == 2, "only the volume-gradient adjoint can skip the geometry");
     constexpr bool TAPS = GEO;        // the march reads volume taps
     extern __shared__ float lds[];
@@ -179,7 +182,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     // the adjoint, its eight volume-gradient contributions).
     auto global_taps = [&](float x, float y, float z) {
         const Cell c = cell_of(b, x, y, z);
-        const int i000 = (c.z0 * b.H + c.y0) * b.W + c.x0;
+        const VoxelIndex i000 = (VoxelIndex)(c.z0 * b.H + c.y0) * b.W + c.x0;   // z0*H + y0 < D*H < 2^31
         const int ox1 = c.sx, oy1 = c.sy * b.W, oz1 = c.sz * plane;
         Taps tp{};
         if (TAPS) tp = load_taps_strided(P.vol, i000, ox1, oy1, oz1);
@@ -509,7 +512,9 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 const int z = (int)fmaf((float)r, rB1, 0.5f * rB1);
                 const int y = min(max(lo[1] + (r - __mul24(z, B1)), 0), b.H - 1);
                 const int zz = min(max(lo[2] + z, 0), b.D - 1);
-                return (int)(__umul24((unsigned)zz, (unsigned)b.H) + (unsigned)y) * b.W + x;
+                // zz*H + y < D*H < 2^31 (host check, D and H below 2^24); the whole offset is
+                // 64-bit when the volume exceeds 2^31 voxels (BIG)
+                return (VoxelIndex)(__umul24((unsigned)zz, (unsigned)b.H) + (unsigned)y) * b.W + x;
             };
             if (staged) {
                 if (!BCAST) __syncthreads();
@@ -548,7 +553,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                     const int ival = (int)gboxu[i];
                     if (ival != 0) {
                         const float gval = (float)ival * inv_scale;
-                        const int vi = box_voxel(i);
+                        const VoxelIndex vi = box_voxel(i);
                         if (P.gacc) {
                             const long long q = (long long)ival;
                             const int sh = E - e_tile;
@@ -668,20 +673,20 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
     }
 }
 
-template <int MODE, bool DIV, bool ZOFF, int CAPC = 0, bool GEO = true>
+template <int MODE, bool DIV, bool ZOFF, int CAPC = 0, bool GEO = true, bool BIG = false>
 __global__ __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 5 : 1)))
 void fp_slab_kernel(SlabArgs P) {
     // Workgroups are dispatched round-robin over the 8 XCDs; permute so that consecutive tiles of a
     // view share an XCD (and its L2).
     const int xcd = blockIdx.x & 7;
     const int tile = ((P.nblocks & 7) == 0) ? xcd * (P.nblocks >> 3) + (blockIdx.x >> 3) : (int)blockIdx.x;
-    slab_tile<MODE, DIV, ZOFF, CAPC, GEO>(P, tile);
+    slab_tile<MODE, DIV, ZOFF, CAPC, GEO, BIG>(P, tile);
 }
 
 // Persistent calibration kernel: workgroups pull tiles of all problems from one counter.
 // MODE 3 (Adam: loss gradient) or 4 (Levenberg-Marquardt: normal equations); one launch serves
 // one solver, so each mode is its own kernel with its own register budget.
-template <int MODE>
+template <int MODE, bool BIG>
 __global__ __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(7))) void fp_slab_batched_kernel(const CalibDev* __restrict__ probs,
                                                                      int nprob, int total_tiles,
                                                                      unsigned* __restrict__ counter) {
@@ -699,18 +704,21 @@ __global__ __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(7)))
         }
         const SlabArgs P = probs[lo].s;
         const int t = tile - probs[lo].tile_offset;
-        if (probs[lo].div) slab_tile<MODE, true>(P, t);
-        else slab_tile<MODE, false>(P, t);
+        if (probs[lo].div) slab_tile<MODE, true, false, 0, true, BIG>(P, t);
+        else slab_tile<MODE, false, false, 0, true, BIG>(P, t);
     }
 }
 
 int launch_fp_slab_batched(const CalibDev* probs, int nprob, int total_tiles, int max_cap, int solver,
-                           unsigned* counter, int grid, hipStream_t stream) {
+                           unsigned* counter, int grid, hipStream_t stream, bool big) {
     const size_t shmem = (size_t)max_cap * sizeof(float);
-    if (solver == 1)
-        fp_slab_batched_kernel<4><<<grid, SL_THREADS, shmem, stream>>>(probs, nprob, total_tiles, counter);
-    else
-        fp_slab_batched_kernel<3><<<grid, SL_THREADS, shmem, stream>>>(probs, nprob, total_tiles, counter);
+    if (solver == 1) {
+        if (big) fp_slab_batched_kernel<4, true><<<grid, SL_THREADS, shmem, stream>>>(probs, nprob, total_tiles, counter);
+        else fp_slab_batched_kernel<4, false><<<grid, SL_THREADS, shmem, stream>>>(probs, nprob, total_tiles, counter);
+    } else {
+        if (big) fp_slab_batched_kernel<3, true><<<grid, SL_THREADS, shmem, stream>>>(probs, nprob, total_tiles, counter);
+        else fp_slab_batched_kernel<3, false><<<grid, SL_THREADS, shmem, stream>>>(probs, nprob, total_tiles, counter);
+    }
     CBCT_LAUNCHED("batched calibration kernel launch");
     return CBCT_OK;
 }
@@ -719,7 +727,15 @@ int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream,
     const size_t shmem = (size_t)args.cap * sizeof(float) * (mode == 2 && geometry ? 2 : 1);
     const dim3 grid((unsigned)args.nblocks), block(SL_THREADS);
     const bool zoff = args.b.offx == 0.f && args.b.offy == 0.f && args.b.offz == 0.f;
-    if (mode == 2 && !geometry) {
+    const bool big = (int64_t)args.b.D * args.b.H * args.b.W > INT32_MAX;
+    if (big) {
+        // 64-bit voxel offsets: one generic variant per mode
+        if (mode == 0 && div) fp_slab_kernel<0, true, false, 0, true, true><<<grid, block, shmem, stream>>>(args);
+        else if (mode == 0) fp_slab_kernel<0, false, false, 0, true, true><<<grid, block, shmem, stream>>>(args);
+        else if (mode == 1) fp_slab_kernel<1, false, false, 0, true, true><<<grid, block, shmem, stream>>>(args);
+        else if (geometry) fp_slab_kernel<2, false, false, 0, true, true><<<grid, block, shmem, stream>>>(args);
+        else fp_slab_kernel<2, false, false, 0, false, true><<<grid, block, shmem, stream>>>(args);
+    } else if (mode == 2 && !geometry) {
         // volume gradient only: no taps, so neither the LDS capacity nor the offsets need to be
         // compile-time constants
         if (zoff) fp_slab_kernel<2, false, true, 0, false><<<grid, block, shmem, stream>>>(args);

@@ -147,6 +147,72 @@ def calibrate(problems, steps=20, lr=DEFAULT_LR, betas=(0.9, 0.999), eps=1e-8, u
     return problems
 
 
+# Parameters of calibrate_autograd: the detector is fitted through its physical shifts
+# (delta_u, delta_v, in length units) and detector_centers, as the reference's docs suggest
+# (docs/USAGE.md:304, cbct_backprojector/operator.py:32-45).
+AUTOGRAD_PARAMETERS = ("angles", "sid", "sod", "delta_u", "delta_v", "src_x", "src_y", "src_z")
+
+
+def detector_shifts(det_u0, det_v0, detector_u, detector_v, pixel_size):
+    """(delta_u, delta_v) of detector centres: the inverse of ``detector_centers``."""
+    return (((detector_u - 1) / 2.0 - det_u0) * pixel_size, ((detector_v - 1) / 2.0 - det_v0) * pixel_size)
+
+
+def calibrate_autograd(volume, measured, geometry, pixel_size, step_size, voxel_size, steps=20, lr=None,
+                       fit=AUTOGRAD_PARAMETERS, betas=(0.9, 0.999), eps=1e-8, volume_offset=(0.0, 0.0, 0.0)):
+    """The plain calibration loop through the public, differentiable API: leaf tensors for the
+    fitted parameters (the detector through its physical shifts delta_u / delta_v, mapped to
+    centre indices by ``detector_centers`` inside the graph), ``project``, the mean squared
+    reprojection residual, ``backward()`` and one ``torch.optim.Adam`` step per iteration, with one
+    learning rate per parameter group.
+
+    ``geometry``: the starting geometry ([8, A] or eight [A] tensors, in the projector's order with
+    det_u0 / det_v0 as centre indices).  ``lr``: learning rates of AUTOGRAD_PARAMETERS (default:
+    DEFAULT_LR with the detector rates converted to length units, pixel-rate x pixel_size).
+    ``fit``: the parameters to fit (the others stay fixed).  Returns (geometry [8, A] as centre
+    indices, {"delta_u": ..., "delta_v": ...}, loss history [steps + 1]).
+
+    Adam divides each gradient by its own running RMS, so fitting delta_u with rate r follows the
+    same trajectory as fitting det_u0 = (U-1)/2 - delta_u / pixel_size with rate r / pixel_size
+    (up to the eps term): the fused batched ``calibrate`` is the fast form of this loop."""
+    from .operator import detector_centers
+    from .source_only_forward_projector import project
+
+    g0 = stack_geometry(geometry).to(volume.device)
+    A, U, V = measured.shape
+    if lr is None:
+        lr = list(DEFAULT_LR)
+        lr[3] *= pixel_size
+        lr[4] *= pixel_size
+    du0, dv0 = detector_shifts(g0[3], g0[4], U, V, pixel_size)
+    start = {"angles": g0[0], "sid": g0[1], "sod": g0[2], "delta_u": du0, "delta_v": dv0,
+             "src_x": g0[5], "src_y": g0[6], "src_z": g0[7]}
+    params = {k: v.detach().clone().requires_grad_(k in fit) for k, v in start.items()}
+    groups = [{"params": [params[k]], "lr": float(lr[i])} for i, k in enumerate(AUTOGRAD_PARAMETERS) if k in fit]
+    opt = torch.optim.Adam(groups, betas=betas, eps=eps)
+    history = []
+
+    def loss_now():
+        u0, v0 = detector_centers(params["delta_u"], params["delta_v"], U, V, pixel_size)
+        proj = project(volume, params["angles"], params["sid"], params["sod"], u0, v0, params["src_x"],
+                       params["src_y"], params["src_z"], pixel_size, U, V, step_size, voxel_size, *volume_offset)
+        return ((proj - measured) ** 2).mean()
+
+    for _ in range(steps):
+        opt.zero_grad(set_to_none=True)
+        loss = loss_now()
+        loss.backward()
+        history.append(loss.detach())
+        opt.step()
+    with torch.no_grad():
+        history.append(loss_now())
+        u0, v0 = detector_centers(params["delta_u"], params["delta_v"], U, V, pixel_size)
+        out = torch.stack([params["angles"], params["sid"], params["sod"], u0, v0, params["src_x"],
+                           params["src_y"], params["src_z"]]).detach()
+    shifts = {"delta_u": params["delta_u"].detach(), "delta_v": params["delta_v"].detach()}
+    return out, shifts, torch.stack(history)
+
+
 def fixed_points(volume_shape, voxel_size, volume_offset=(0.0, 0.0, 0.0)):
     """The volume's eight corner voxel centres and its centre, world coordinates [9, 3] (x, y, z)."""
     D, H, W = volume_shape

@@ -125,3 +125,13 @@ def test_autotune_cache_round_trip_and_shape_classes(tmp_path, monkeypatch):
     batch = autotune.batch_candidates()
     assert batch and set(batch) <= set(autotune.candidates())
     assert max(c[1] for c in batch) == autotune.BATCH_MAX_LDS_FLOATS
+
+
+def test_detector_shifts_invert_detector_centers():
+    import torch
+    from cbct_backprojector import calibration, detector_centers
+    du = torch.tensor([0.3, -1.2, 0.0], dtype=torch.float64)
+    dv = torch.tensor([-0.7, 0.25, 2.0], dtype=torch.float64)
+    u0, v0 = detector_centers(du, dv, 48, 40, 0.8)
+    su, sv = calibration.detector_shifts(u0, v0, 48, 40, 0.8)
+    assert torch.allclose(su, du, atol=1e-12) and torch.allclose(sv, dv, atol=1e-12)

@@ -22,6 +22,10 @@ Prints ONE JSON line:
              sample + 4 B per ray + 4 B per voxel) over the duration, as a ratio to 8 TB/s
              (above 1 because LDS serves most taps: it is not a roof).
   cpu_baseline  the FP32 CPU oracle (OpenMP) on a bounded subset of views.
+  calibration   the metric's second half (default workload, unless --no-calibration): seconds
+             to 0.1 px reprojection error on 256^3 x 180-view scans from the learned warm start
+             and from the plain start, for Adam and for Levenberg-Marquardt, with the network
+             trained in the run (about 45 s; not counted in the times).
 """
 
 import argparse
@@ -52,8 +56,10 @@ CALIB_WORKLOAD = "batched-calibration-64p"
 WORKLOADS[CALIB_WORKLOAD] = dict(problems=64, adam_steps=20, side=(96, 320), views=(90, 720), detector=(128, 768),
                                  voxel=0.5, sod=600.0, sid=1100.0, seed=4)
 WARM_WORKLOAD = "calibration-time-256c-180v"
-WORKLOADS[WARM_WORKLOAD] = dict(train_steps=1500, pool=24, batch=2, scans=(7001, 7002, 7003, 7004), threshold=0.1,
-                                adam_budget=150, lm_budget=20, plain_adam_lr=4.0, warm_adam_lr=1.0, lm_lambda=1e-3)
+# one Adam schedule for both starts: calibration.DEFAULT_LR, constant (multiplier, decay per step)
+WORKLOADS[WARM_WORKLOAD] = dict(train_steps=1500, pool=24, batch=2, scans=(7002, 7003, 7004), threshold=0.1,
+                                passes_adam=3, passes_lm=1, adam_budget=2000, lm_budget=20, lm_lambda=1e-3,
+                                adam=(1.0, 1.0))
 BP_WORKLOAD = "backprojection-256c-360v"
 WORKLOADS[BP_WORKLOAD] = dict(D=256, A=360, U=512, V=384, pixel=0.8, sod=500.0, sid=1000.0, voxel=0.5, step=0.25,
                               perturb=True)
@@ -147,7 +153,9 @@ def parse():
     p.add_argument("--deterministic", action="store_true",
                    help="bitwise-reproducible volume / projection gradients (int64 fixed-point accumulation)")
     p.add_argument("--scans", type=int, default=None,
-                   help="held-out scans of the calibration-time workload (default 4)")
+                   help="held-out scans of the calibration-time workload (default 3)")
+    p.add_argument("--no-calibration", action="store_true",
+                   help="leave the calibration-time object out of the default workload's line")
     p.add_argument("--fp-tuning", default=None,
                    help="projector kernels as use_slab,slab_thickness,lds_floats,layout (default: library's)")
     return p.parse_args()
@@ -400,15 +408,18 @@ def cpu_baseline_calibration(probs, budget_s):
                       f"(FP32 C oracle), {t_total:.1f} s"}
 
 
-def warm_start_bench(args, torch, device):
-    """Seconds to 0.1 px reprojection geometry error on 256^3 x 180-view scans (detector
-    512^2, pixel 0.8 mm, geometry perturbed as in the 256^3 x 720 workload), learned warm start
-    vs plain start.  The network is first trained here on projector-generated scans (its
-    training time is reported, not counted).  For each held-out scan and each method the number
-    of iterations to reach the threshold is found first, untimed; then exactly that many
-    iterations run again from the same start inside one timed call (warm start included for
-    the learned methods).  Methods: plain Adam, learned + Adam, plain Levenberg-Marquardt,
-    learned + Levenberg-Marquardt."""
+def calibration_time(args, torch, device):
+    """Seconds to 0.1 px reprojection geometry error on 256^3 x 180-view scans (detector 512^2,
+    pixel 0.8 mm, geometry perturbed as in the 256^3 x 720 workload): learned warm start against
+    the plain (nominal) start, for each solver.  The network is trained here first on
+    projector-generated scans (training time reported, not counted).  For each held-out scan and
+    method the iteration count to the threshold is found first, untimed (Adam in 10-step
+    chunks); then exactly that many iterations run again from the same start inside one timed
+    call, the network passes included for the learned methods.  Both starts run the same Adam
+    schedule (the default per-group learning rates, constant: decaying schedules stall short of
+    the threshold on some scans) and the same Levenberg-Marquardt damping; the network is
+    applied three times before Adam and once before Levenberg-Marquardt, whose own iterations
+    cost about as much as a network pass.  Returns the calibration object."""
     from cbct_backprojector import calibration as C
     from cbct_backprojector import warmstart as ws
     w = WORKLOADS[WARM_WORKLOAD]
@@ -418,11 +429,12 @@ def warm_start_bench(args, torch, device):
     t0 = time.perf_counter()
     data = ws.TrainingData(fam, pool=w["pool"], seed=1000, device=device)
     net = ws.WarmStartNet(seed=0, device=device)
-    hist = ws.train(net, data, steps_train, P=w["batch"], lr=1e-3, lr_final=1e-5, log_every=250, log=log)
+    hist = ws.train(net, data, steps_train, P=w["batch"], lr=1e-3, lr_final=1e-5, log_every=500, log=log)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     del data
     thr = w["threshold"]
+    sched = w["adam"]
     err = lambda p: C.reprojection_error_px(p.geometry, p.true_geometry, tuple(p.volume.shape), p.voxel_size,
                                             p.pixel_size)
 
@@ -431,25 +443,28 @@ def warm_start_bench(args, torch, device):
         p.adam_m.zero_()
         p.adam_v.zero_()
 
-    def adam_iters(p, g, mult):
+    def adam_lr(sched):
+        return [x * sched[0] for x in C.DEFAULT_LR]
+
+    def adam_iters(p, g, sched):
         reset(p, g)
-        lr = [x * mult for x in C.DEFAULT_LR]
-        for t in range(w["adam_budget"]):
-            C.calibrate([p], steps=1, lr=lr, first_step=t, use_graph=False)
+        chunk = 10
+        for t in range(0, w["adam_budget"], chunk):
+            C.calibrate([p], steps=chunk, lr=adam_lr(sched), lr_decay=sched[1], first_step=t, use_graph=False)
             if err(p) < thr:
-                return t + 1, err(p)
-        return None, err(p)
+                return t + chunk
+        return None
 
     def lm_iters(p, g):
         for t in range(1, w["lm_budget"] + 1):
             reset(p, g)
             C.calibrate([p], steps=t, solver="lm", lm_lambda=w["lm_lambda"])
             if err(p) < thr:
-                return t, err(p)
-        return None, err(p)
+                return t
+        return None
 
-    def timed(p, g_start, n, solver, mult, warm):
-        reset(p, g_start)
+    def timed(p, g_nom, n, solver, sched, warm):
+        reset(p, g_nom)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         if warm:
@@ -458,7 +473,7 @@ def warm_start_bench(args, torch, device):
             if solver == "lm":
                 C.calibrate([p], steps=n, solver="lm", lm_lambda=w["lm_lambda"])
             else:
-                C.calibrate([p], steps=n, lr=[x * mult for x in C.DEFAULT_LR])
+                C.calibrate([p], steps=n, lr=adam_lr(sched), lr_decay=sched[1])
         torch.cuda.synchronize()
         return time.perf_counter() - t0, err(p)
 
@@ -466,59 +481,55 @@ def warm_start_bench(args, torch, device):
     for seed in w["scans"][:args.scans]:
         p = C.synthetic_problem(fam.spec(seed), device)
         g_nom = p.geometry.clone()
-        e_nom = err(p)
-        g_warm, e_warm = {}, {}
-        for k in (1, 2, 3):            # refinement passes of the network
+        row = {"scan": seed, "px_nominal": err(p), "px_by_network_passes": []}
+        g_warm = {}
+        for k in range(1, max(w["passes_adam"], w["passes_lm"]) + 1):
             reset(p, g_nom)
             ws.warm_start(p, net, fam, iterations=k)
-            g_warm[k], e_warm[k] = p.geometry.clone(), err(p)
-        row = {"scan": seed, "px_nominal": e_nom, "px_warm_start": e_warm}
-        methods = [("plain_adam", g_nom, "adam", w["plain_adam_lr"], 0),
-                   ("learned_adam", g_warm[3], "adam", w["warm_adam_lr"], 3), ("plain_lm", g_nom, "lm", None, 0)]
-        methods += [(f"learned{k}_lm", g_warm[k], "lm", None, k) for k in (1, 2, 3)]
-        for name, start, solver, mult, warm in methods:
-            n, e_end = adam_iters(p, start, mult) if solver == "adam" else lm_iters(p, start)
-            budget = w["adam_budget"] if solver == "adam" else w["lm_budget"]
-            t, e_t = timed(p, g_nom, n if n else budget, solver, mult, warm)
-            row[name] = {"iterations": n, "s": t if n else None, "px": e_t,
-                         "s_budget_unreached": None if n else t}
-            log(f"scan {seed} {name}: {n} iterations, {t:.3f} s, {e_t:.3f} px")
+            row["px_by_network_passes"].append(err(p))
+            g_warm[k] = p.geometry.clone()
+        methods = [("plain_adam", g_nom, "adam", 0), ("learned_adam", g_warm[w["passes_adam"]], "adam", w["passes_adam"]),
+                   ("plain_lm", g_nom, "lm", 0), ("learned_lm", g_warm[w["passes_lm"]], "lm", w["passes_lm"])]
+        for name, start, solver, warm in methods:
+            n = adam_iters(p, start, sched) if solver == "adam" else lm_iters(p, start)
+            t, e_t = timed(p, g_nom, n, solver, sched, warm) if n else (None, None)
+            row[name] = {"iterations": n, "s": t, "px": e_t}
+            log(f"scan {seed} {name}: {n} iterations, {t} s, {e_t} px")
         rows.append(row)
 
     def med(key):
         v = [r[key]["s"] for r in rows]
-        return None if any(x is None for x in v) else sorted(v)[len(v) // 2]
+        return None if not v or any(x is None for x in v) else sorted(v)[len(v) // 2]
 
-    def med_at_least(key):
-        # time to the threshold, or the whole budget's time where the threshold was not reached:
-        # a lower bound on the time to the threshold
-        v = [r[key]["s"] if r[key]["s"] is not None else r[key]["s_budget_unreached"] for r in rows]
-        return sorted(v)[len(v) // 2]
-
-    t_plain_adam, t_learned_adam = med("plain_adam"), med("learned_adam")
-    t_plain_lm = med("plain_lm")
-    learned_lm = {k: med(f"learned{k}_lm") for k in (1, 2, 3)}
-    best_k = min((k for k in learned_lm if learned_lm[k] is not None), key=lambda k: learned_lm[k], default=3)
-    t_learned_lm = learned_lm[best_k]
     ratio = lambda a, b: None if a is None or b is None else a / b
+    s = {k: med(k) for k in ("plain_adam", "learned_adam", "plain_lm", "learned_lm")}
+
+    def fastest(start):
+        done = [(v, k) for k, v in s.items() if k.startswith(start) and v is not None]
+        return min(done)[1] if done else None
+
+    return {"workload": WARM_WORKLOAD, "unit": "s to 0.1 px (median over held-out scans)", "threshold_px": thr,
+            "volume": [fam.D] * 3, "views": fam.A, "detector": [fam.U, fam.V], "pixel_mm": fam.pixel,
+            "median_s": s,
+            "speedup_same_solver": {"adam": ratio(s["plain_adam"], s["learned_adam"]),
+                                    "lm": ratio(s["plain_lm"], s["learned_lm"])},
+            "fastest": {start: fastest(start) for start in ("plain", "learned")},
+            "network": {"parameters": net.num_parameters,
+                        "passes": {"adam": w["passes_adam"], "lm": w["passes_lm"]}, "train_steps": steps_train,
+                        "train_s": train_s, "train_loss_first_last": [sum(hist[:50]) / 50, sum(hist[-50:]) / 50],
+                        "dtype": "bf16 GEMMs, f32 accumulation"},
+            "adam_schedule": {"lr": [x * sched[0] for x in C.DEFAULT_LR], "decay_per_step": sched[1],
+                              "budget_steps": w["adam_budget"]},
+            "lm_lambda": w["lm_lambda"], "scans": rows}
+
+
+def warm_start_bench(args, torch, device):
+    """The calibration-time workload alone: its object as the JSON line's config."""
+    c = calibration_time(args, torch, device)
     out = {
-        "metric": METRIC, "value": t_learned_lm, "unit": "s to 0.1 px (median over scans)", "n_gpus": 1,
-        "steps": len(rows), "warmup": 0, "ms_per_step": None, "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 projector / bf16 network", "data": "synthetic",
-        "config": {"workload": WARM_WORKLOAD, "volume": [fam.D] * 3, "views": fam.A, "detector": [fam.U, fam.V],
-                   "pixel_mm": fam.pixel, "threshold_px": thr, "train_steps": steps_train, "train_s": train_s,
-                   "train_loss_first_last": [sum(hist[:50]) / 50, sum(hist[-50:]) / 50],
-                   "network_parameters": net.num_parameters,
-                   "median_s": {"plain_adam": t_plain_adam, "learned_adam": t_learned_adam,
-                                "plain_lm": t_plain_lm, "learned_lm": t_learned_lm,
-                                "learned_lm_by_network_passes": learned_lm},
-                   "network_passes": best_k,
-                   "speedup_learned_lm_vs_plain_adam": ratio(t_plain_adam, t_learned_lm),
-                   "speedup_learned_lm_vs_plain_adam_at_least": ratio(med_at_least("plain_adam"), t_learned_lm),
-                   "speedup_learned_lm_vs_plain_lm": ratio(t_plain_lm, t_learned_lm),
-                   "speedup_learned_adam_vs_plain_adam": ratio(t_plain_adam, t_learned_adam),
-                   "adam_lr_multipliers": {"plain": w["plain_adam_lr"], "learned": w["warm_adam_lr"]},
-                   "scans": rows},
+        "metric": METRIC, "value": c["median_s"]["learned_lm"], "unit": c["unit"], "n_gpus": 1,
+        "steps": len(c["scans"]), "warmup": 0, "ms_per_step": None, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 projector / bf16 network", "data": "synthetic", "config": c,
     }
     print(json.dumps(out), flush=True)
 
@@ -735,6 +746,11 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = (cpu_baseline_fdk(w, vol_np, geom_np, args.cpu_seconds) if fdk
                                else cpu_baseline(w, vol_np, geom_np, args.cpu_seconds))
+    if args.workload == DEFAULT_WORKLOAD and not args.no_calibration:
+        # the metric's second half: seconds to 0.1 px, learned vs plain start, per solver
+        del vol, geom, upstream
+        torch.cuda.empty_cache()
+        out["calibration"] = calibration_time(args, torch, device)
     print(json.dumps(out), flush=True)
 
 

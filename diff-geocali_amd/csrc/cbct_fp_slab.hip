@@ -516,10 +516,63 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 // 64-bit when the volume exceeds 2^31 voxels (BIG)
                 return (VoxelIndex)(__umul24((unsigned)zz, (unsigned)b.H) + (unsigned)y) * b.W + x;
             };
+            // A box inside the volume needs no clamp, and a lane's elements i, i + 256, ... can be
+            // walked: from element i to i + 256 the x coordinate moves by rem = 256 mod B0 (one
+            // wrap at most, carrying into the row) and the row by q = 256 / B0 or q + 1 (one wrap
+            // of y at most when q + 1 <= B1, carrying into z), so the voxel offset moves by one of
+            // four uniform constants.  By induction the walked (x, y, offset) equal the element's
+            // own decomposition for every element of the box.  Used by the volume-gradient adjoint
+            // (staging and flush: 164 -> 161 ms at 256^3 x 720), not by the gathering kernels,
+            // whose forward it slows by about 1 ms.
+            const bool walk = MODE == 2 && staged && lo[0] >= 0 && lo[1] >= 0 && lo[2] >= 0 && hi[0] < b.W && hi[1] < b.H &&
+                              hi[2] < b.D && SL_THREADS / B0 + 1 <= B1;
+            const int wq = SL_THREADS / B0, wrem = SL_THREADS - wq * B0;
+            const VoxelIndex plane_v = (VoxelIndex)b.H * b.W;
+            const VoxelIndex wk0 = (VoxelIndex)wrem + (VoxelIndex)wq * b.W, wkx = (VoxelIndex)(b.W - B0);
+            const VoxelIndex wky = plane_v - (VoxelIndex)B1 * b.W;
+            int wx = 0, wy = 0;
+            VoxelIndex woff = 0;
+            auto walk_start = [&]() {
+                const int i = threadIdx.x;
+                const int r = (int)fmaf((float)i, rB0, 0.5f * rB0);
+                const int z = (int)fmaf((float)r, rB1, 0.5f * rB1);
+                wx = i - __mul24(r, B0);
+                wy = r - __mul24(z, B1);
+                woff = ((VoxelIndex)(lo[2] + z) * b.H + (lo[1] + wy)) * b.W + lo[0] + wx;
+            };
+            auto walk_next = [&]() {
+                wx += wrem;
+                const bool cx = wx >= B0;
+                wx = cx ? wx - B0 : wx;
+                wy += cx ? wq + 1 : wq;
+                const bool cy = wy >= B1;
+                wy = cy ? wy - B1 : wy;
+                woff += wk0 + (cx ? wkx : (VoxelIndex)0) + (cy ? wky : (VoxelIndex)0);
+            };
             if (staged) {
                 if (!BCAST) __syncthreads();
                 // four loads in flight per lane before their LDS stores
                 constexpr int NB = 4;
+                if (walk) {
+                    walk_start();
+                    for (int i0 = threadIdx.x; i0 < nbox; i0 += NB * SL_THREADS) {
+                        float vv[NB];
+#pragma unroll
+                        for (int k = 0; k < NB; ++k) {
+                            const int i = i0 + k * SL_THREADS;
+                            vv[k] = TAPS && i < nbox ? P.vol[woff] : 0.f;
+                            walk_next();
+                        }
+#pragma unroll
+                        for (int k = 0; k < NB; ++k) {
+                            const int i = i0 + k * SL_THREADS;
+                            if (i < nbox) {
+                                if (TAPS) vbox[i] = vv[k];
+                                if (MODE == 2) gboxu[i] = 0u;
+                            }
+                        }
+                    }
+                } else
                 for (int i0 = threadIdx.x; i0 < nbox; i0 += NB * SL_THREADS) {
                     float vv[NB];
 #pragma unroll
@@ -549,19 +602,27 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             }
             if (MODE == 2 && staged) {
                 __syncthreads();
-                for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {
-                    const int ival = (int)gboxu[i];
-                    if (ival != 0) {
-                        const float gval = (float)ival * inv_scale;
-                        const VoxelIndex vi = box_voxel(i);
-                        if (P.gacc) {
-                            const long long q = (long long)ival;
-                            const int sh = E - e_tile;
-                            const long long iv = sh >= 0 ? (q << min(sh, 62)) : (q >> min(-sh, 62));
-                            if (iv != 0) atomicAdd(P.gacc + vi, (unsigned long long)iv);
-                        } else {
-                            atomicAdd(P.gvol + vi, gval);
-                        }
+                auto flush = [&](int ival, VoxelIndex vi) {
+                    if (P.gacc) {
+                        const long long q = (long long)ival;
+                        const int sh = E - e_tile;
+                        const long long iv = sh >= 0 ? (q << min(sh, 62)) : (q >> min(-sh, 62));
+                        if (iv != 0) atomicAdd(P.gacc + vi, (unsigned long long)iv);
+                    } else {
+                        atomicAdd(P.gvol + vi, (float)ival * inv_scale);
+                    }
+                };
+                if (walk) {
+                    walk_start();
+                    for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {
+                        const int ival = (int)gboxu[i];
+                        if (ival != 0) flush(ival, woff);
+                        walk_next();
+                    }
+                } else {
+                    for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {
+                        const int ival = (int)gboxu[i];
+                        if (ival != 0) flush(ival, box_voxel(i));
                     }
                 }
             }

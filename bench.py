@@ -73,8 +73,8 @@ DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 # KiB of FETCH_SIZE and 1,048,576 KiB of WRITE_SIZE, so HBM bytes = FETCH_SIZE x 2.0 + WRITE_SIZE x
 # 1.0 (KiB x 1024).  The counts depend only on the workload, the kernel and its tuning.
 MEASURED = {
-    ("geometry-gradients-256c-720v", "adjoint"): dict(valu=111_882_127_715, lds_conflict=38_899_405_936,
-                                                      grbm=3_130_928_217, fetch_kib=103_288_638, write_kib=137_723_562),
+    ("geometry-gradients-256c-720v", "adjoint"): dict(valu=112_309_772_582, lds_conflict=38_899_405_936,
+                                                      grbm=3_062_420_922, fetch_kib=105_531_316, write_kib=138_734_132),
     ("geometry-gradients-256c-720v", "forward"): dict(valu=54_536_859_549, lds_conflict=9_670_306_160,
                                                       grbm=1_271_345_956, fetch_kib=91_349_087, write_kib=1_466_077),
     # forward at 360 views, 512x384 (FDK round trip); backprojection backward / forward at 256^3 x 360

@@ -77,6 +77,9 @@ MEASURED = {
                                                       grbm=3_062_420_922, fetch_kib=105_531_316, write_kib=138_734_132),
     ("geometry-gradients-256c-720v", "forward"): dict(valu=54_536_859_549, lds_conflict=9_670_306_160,
                                                       grbm=1_271_345_956, fetch_kib=91_349_087, write_kib=1_466_077),
+    # one fused Adam march of the 64-problem batched calibration (all problems, one launch)
+    (CALIB_WORKLOAD, "march"): dict(valu=1_274_549_605_032, lds_conflict=196_736_971_792, grbm=30_497_691_569,
+                                    fetch_kib=3_232_548_829, write_kib=58_107_987),
     # forward at 360 views, 512x384 (FDK round trip); backprojection backward / forward at 256^3 x 360
     # views (HBM counters only)
     ("fdk-roundtrip-256c-360v", "forward"): dict(fetch_kib=47_640_337, write_kib=296_104),

@@ -526,19 +526,23 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             // whose forward it slows by about 1 ms.
             const bool walk = MODE == 2 && staged && lo[0] >= 0 && lo[1] >= 0 && lo[2] >= 0 && hi[0] < b.W && hi[1] < b.H &&
                               hi[2] < b.D && SL_THREADS / B0 + 1 <= B1;
+            // (unsigned: a lane's walk runs a few elements past the box's last one, whose offsets
+            // are never used and may exceed the 32-bit range of the offsets in use)
+            using WalkIndex = typename std::make_unsigned<VoxelIndex>::type;
             const int wq = SL_THREADS / B0, wrem = SL_THREADS - wq * B0;
-            const VoxelIndex plane_v = (VoxelIndex)b.H * b.W;
-            const VoxelIndex wk0 = (VoxelIndex)wrem + (VoxelIndex)wq * b.W, wkx = (VoxelIndex)(b.W - B0);
-            const VoxelIndex wky = plane_v - (VoxelIndex)B1 * b.W;
+            const WalkIndex plane_v = (WalkIndex)b.H * (WalkIndex)b.W;
+            const WalkIndex wk0 = (WalkIndex)wrem + (WalkIndex)wq * (WalkIndex)b.W, wkx = (WalkIndex)(b.W - B0);
+            const WalkIndex wky = plane_v - (WalkIndex)B1 * (WalkIndex)b.W;
             int wx = 0, wy = 0;
-            VoxelIndex woff = 0;
+            WalkIndex woff = 0;
             auto walk_start = [&]() {
                 const int i = threadIdx.x;
                 const int r = (int)fmaf((float)i, rB0, 0.5f * rB0);
                 const int z = (int)fmaf((float)r, rB1, 0.5f * rB1);
                 wx = i - __mul24(r, B0);
                 wy = r - __mul24(z, B1);
-                woff = ((VoxelIndex)(lo[2] + z) * b.H + (lo[1] + wy)) * b.W + lo[0] + wx;
+                woff = ((WalkIndex)(lo[2] + z) * (WalkIndex)b.H + (WalkIndex)(lo[1] + wy)) * (WalkIndex)b.W +
+                       (WalkIndex)(lo[0] + wx);
             };
             auto walk_next = [&]() {
                 wx += wrem;
@@ -547,7 +551,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 wy += cx ? wq + 1 : wq;
                 const bool cy = wy >= B1;
                 wy = cy ? wy - B1 : wy;
-                woff += wk0 + (cx ? wkx : (VoxelIndex)0) + (cy ? wky : (VoxelIndex)0);
+                woff += wk0 + (cx ? wkx : (WalkIndex)0) + (cy ? wky : (WalkIndex)0);
             };
             if (staged) {
                 if (!BCAST) __syncthreads();
@@ -560,7 +564,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
 #pragma unroll
                         for (int k = 0; k < NB; ++k) {
                             const int i = i0 + k * SL_THREADS;
-                            vv[k] = TAPS && i < nbox ? P.vol[woff] : 0.f;
+                            vv[k] = TAPS && i < nbox ? P.vol[(VoxelIndex)woff] : 0.f;
                             walk_next();
                         }
 #pragma unroll
@@ -616,7 +620,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                     walk_start();
                     for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {
                         const int ival = (int)gboxu[i];
-                        if (ival != 0) flush(ival, woff);
+                        if (ival != 0) flush(ival, (VoxelIndex)woff);
                         walk_next();
                     }
                 } else {

@@ -10,17 +10,18 @@ gradients) for a fixed upstream gradient, inputs resident in HBM.
 
 Prints ONE JSON line:
   value      GUPS = 2*D*H*W*A / (t_fwd + t_adj) / 1e9
-  roofline   the roof the dominant kernel is measured against.  The projector kernels are
-             bound by VALU issue (the counters show it nearest its roof; HBM and LDS are
-             further from theirs): achieved = SQ_INSTS_VALU per launch (a constant of the
-             workload and kernel, from rocprofv3 --pmc, kept below) x 64 lanes / the
-             kernel's HIP-event duration measured here; peak = 2 wave instructions per
-             CU-cycle x 64 lanes x 256 CUs x 2.4 GHz = 78.6 TOP/s (the 157.3 TFLOP/s FP32
-             vector peak counted per lane-instruction).  Beside it: the measured HBM traffic
-             and its fraction of 8 TB/s, the LDS bank-conflict share, and the tap model's
-             bytes (forward: 32 B per trilinear sample + 4 B per ray; adjoint: 64 B per
-             sample + 4 B per ray + 4 B per voxel) over the duration, as a ratio to 8 TB/s
-             (above 1 because LDS serves most taps: it is not a roof).
+  roofline   the roof the dominant kernel is measured against: VALU issue.  achieved =
+             SQ_INSTS_VALU per launch (a constant of the workload and kernel, from rocprofv3
+             --pmc, kept below) x 64 lanes / the kernel's HIP-event duration measured here;
+             peak = 2 wave instructions per CU-cycle x 64 lanes x 256 CUs x 2.4 GHz = 78.6
+             TOP/s (the 157.3 TFLOP/s FP32 vector peak counted per lane-instruction).  Beside
+             it: the measured HBM traffic and its fraction of 8 TB/s; the LDS pipe's busy
+             share, its bank-conflict share and the share of wave time spent waiting to issue
+             an LDS instruction (the adjoint keeps the LDS busier than the VALU: both are
+             reported); and the tap model's bytes (forward: 32 B per trilinear sample + 4 B
+             per ray; adjoint: 64 B per sample + 4 B per ray + 4 B per voxel) over the
+             duration, as a ratio to 8 TB/s (above 1 because LDS serves most taps: it is not
+             a roof).
   cpu_baseline  the FP32 CPU oracle (OpenMP) on a bounded subset of views.
   calibration   the metric's second half (default workload, unless --no-calibration): seconds
              to 0.1 px reprojection error on 256^3 x 180-view scans from the learned warm start
@@ -67,16 +68,22 @@ PREP_WORKLOAD = "preprocess-720v-768x640-u16"
 WORKLOADS[PREP_WORKLOAD] = dict(A=720, U=768, V=640)
 DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 # Counters per launch of the projector kernels, from rocprofv3 --pmc runs of `bench.py --pmc-probe`
-# (one forward, one adjoint; summaries and commands in profiles/): SQ_INSTS_VALU (wave
-# instructions), SQ_LDS_BANK_CONFLICT (cycles, summed over CUs), GRBM_GUI_ACTIVE (cycles, summed
-# over the 8 XCDs), FETCH_SIZE and WRITE_SIZE (KiB).  A 1 GiB copy in the same runs reads 524,306
-# KiB of FETCH_SIZE and 1,048,576 KiB of WRITE_SIZE, so HBM bytes = FETCH_SIZE x 2.0 + WRITE_SIZE x
-# 1.0 (KiB x 1024).  The counts depend only on the workload, the kernel and its tuning.
+# (summaries and commands in profiles/): SQ_INSTS_VALU (wave instructions), SQ_LDS_BANK_CONFLICT and
+# SQ_LDS_IDX_ACTIVE (LDS cycles stalled by bank conflicts / busy, summed over CUs), SQ_WAIT_INST_LDS
+# and SQ_WAVE_CYCLES (wave quad-cycles waiting to issue an LDS instruction / resident), GRBM_GUI_ACTIVE
+# (cycles, summed over the 8 XCDs), FETCH_SIZE and WRITE_SIZE (KiB).  A 1 GiB copy in the same runs
+# reads 524,306 KiB of FETCH_SIZE and 1,048,576 KiB of WRITE_SIZE, so HBM bytes = FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0 (KiB x 1024).  The counts depend only on the workload, the kernel and its tuning
+# (SQ_WAIT_INST_LDS and SQ_WAVE_CYCLES vary by about 1% between runs).
 MEASURED = {
     ("geometry-gradients-256c-720v", "adjoint"): dict(valu=112_309_772_582, lds_conflict=38_899_405_936,
-                                                      grbm=3_062_420_922, fetch_kib=105_531_316, write_kib=138_734_132),
+                                                      lds_active=74_130_204_968, lds_wait=53_237_905_916,
+                                                      wave_qcycles=456_166_095_225, grbm=3_038_617_749,
+                                                      fetch_kib=105_302_061, write_kib=138_735_632),
     ("geometry-gradients-256c-720v", "forward"): dict(valu=54_536_859_549, lds_conflict=9_670_306_160,
-                                                      grbm=1_271_345_956, fetch_kib=91_349_087, write_kib=1_466_077),
+                                                      lds_active=27_012_445_724, lds_wait=12_673_369_010,
+                                                      wave_qcycles=270_325_842_926, grbm=1_283_814_838,
+                                                      fetch_kib=91_412_406, write_kib=1_465_855),
     # one fused Adam march of the 64-problem batched calibration (all problems, one launch)
     (CALIB_WORKLOAD, "march"): dict(valu=1_274_549_605_032, lds_conflict=196_736_971_792, grbm=30_497_691_569,
                                     fetch_kib=3_232_548_829, write_kib=58_107_987),
@@ -108,9 +115,15 @@ def roofline(workload, kernel, seconds, tap_bytes, measured_ok=True, copy_gbps=N
            "copy_GBps": copy_gbps}
     if m is not None and "valu" in m:
         achieved = m["valu"] * 64 / seconds / 1e12
+        cu_cycles = CUS * m["grbm"] / XCDS
         out.update(bound="valu", achieved=achieved, peak=VALU_PEAK_TOPS, unit="TOP/s (VALU lane-instructions)",
                    frac=achieved / VALU_PEAK_TOPS, valu_insts_per_launch=m["valu"],
-                   lds_bank_conflict_share=m["lds_conflict"] / CUS / (m["grbm"] / XCDS))
+                   lds_bank_conflict_share=m["lds_conflict"] / cu_cycles)
+        if "lds_active" in m:
+            # the LDS pipe's busy share of the launch and the share of resident wave time spent waiting
+            # to issue an LDS instruction
+            out.update(lds_active_share=m["lds_active"] / cu_cycles,
+                       lds_issue_wait_share=m["lds_wait"] / m["wave_qcycles"])
     elif traffic is not None:
         achieved = traffic / seconds / 1e9
         out.update(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBPS, unit="GB/s", frac=achieved / HBM_PEAK_GBPS)
@@ -146,7 +159,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     p.add_argument("--pmc-probe", action="store_true",
-                   help="run a 1 GiB reference copy, one forward and one adjoint (for rocprofv3 --pmc runs) and exit")
+                   help="run a 1 GiB reference copy, one forward, one adjoint and one adjoint without geometry "
+                        "gradients (for rocprofv3 --pmc runs) and exit")
     p.add_argument("--adam-steps", type=int, default=None,
                    help="Adam steps per problem of the batched-calibration workload (default 20)")
     p.add_argument("--solver", default="adam", choices=("adam", "lm"),
@@ -668,6 +682,9 @@ def main():
         del a, b
         _native.fp_forward(vol, geom, *scal)
         _native.fp_backward(upstream, vol, geom, w["pixel"], w["step"], w["voxel"], True, 0.0, 0.0, 0.0)
+        # the adjoint without geometry gradients (volume scatter only), for the bound analysis
+        _native.fp_backward(upstream, vol, geom, w["pixel"], w["step"], w["voxel"], True, 0.0, 0.0, 0.0,
+                            compute_grad_geometry=False)
         torch.cuda.synchronize()
         print("pmc probe done", flush=True)
         return

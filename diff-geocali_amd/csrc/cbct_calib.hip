@@ -260,36 +260,81 @@ struct GraphKey {
 };
 
 constexpr size_t GRAPH_CACHE_ENTRIES = 8;
+// A cached graph and an event recorded after its latest launch.
+struct CachedGraph {
+    GraphKey key;
+    hipGraphExec_t exec;
+    hipEvent_t done;
+};
+// Cached graphs (oldest first) and graphs evicted from the cache.  A graph is launched, and its
+// event recorded, while the mutex is held (both only enqueue work); an evicted graph is destroyed
+// only once its event reports that its last launch has completed (checked without waiting at later
+// insertions) or by cbct_calib_release_graphs, so no graph is freed while another thread launches
+// it or while it still runs.
 static std::mutex g_graph_mutex;
-static std::vector<std::pair<GraphKey, hipGraphExec_t>> g_graphs;   // oldest first
+static std::vector<CachedGraph> g_graphs;
+static std::vector<CachedGraph> g_evicted;
 
 static bool same_key(const GraphKey& a, const GraphKey& b) { return std::memcmp(&a, &b, sizeof(GraphKey)) == 0; }
 
-static hipGraphExec_t graph_cache_find(const GraphKey& k) {
-    std::lock_guard<std::mutex> lock(g_graph_mutex);
-    for (auto& e : g_graphs)
-        if (same_key(e.first, k)) return e.second;
-    return nullptr;
+static hipError_t launch_recorded(const CachedGraph& g, hipStream_t stream) {
+    const hipError_t e = hipGraphLaunch(g.exec, stream);
+    return e != hipSuccess ? e : hipEventRecord(g.done, stream);
 }
 
-// An evicted graph that is still running is freed when it completes (hipGraphExecDestroy).
-static void graph_cache_insert(const GraphKey& k, hipGraphExec_t exec) {
+static void destroy_cached(const CachedGraph& g, hipError_t& first) {
+    hipError_t r = hipGraphExecDestroy(g.exec);
+    if (first == hipSuccess) first = r;
+    r = hipEventDestroy(g.done);
+    if (first == hipSuccess) first = r;
+}
+
+// Launches the cached graph of key k on stream (true), or returns false when there is none.
+static bool graph_cache_launch(const GraphKey& k, hipStream_t stream, hipError_t& err) {
     std::lock_guard<std::mutex> lock(g_graph_mutex);
+    for (auto& g : g_graphs)
+        if (same_key(g.key, k)) {
+            err = launch_recorded(g, stream);
+            return true;
+        }
+    return false;
+}
+
+// Caches a newly instantiated graph (evicting the oldest beyond GRAPH_CACHE_ENTRIES), launches it,
+// and frees the evicted graphs whose last launch has completed.
+static hipError_t graph_cache_insert_launch(const GraphKey& k, hipGraphExec_t exec, hipStream_t stream) {
+    hipEvent_t done = nullptr;
+    hipError_t e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        (void)hipGraphExecDestroy(exec);
+        return e;
+    }
+    std::lock_guard<std::mutex> lock(g_graph_mutex);
+    hipError_t first = hipSuccess;
+    for (size_t i = 0; i < g_evicted.size();) {
+        if (hipEventQuery(g_evicted[i].done) == hipSuccess) {
+            destroy_cached(g_evicted[i], first);
+            g_evicted.erase(g_evicted.begin() + (long)i);
+        } else {
+            ++i;
+        }
+    }
     if (g_graphs.size() >= GRAPH_CACHE_ENTRIES) {
-        (void)hipGraphExecDestroy(g_graphs.front().second);
+        g_evicted.push_back(g_graphs.front());
         g_graphs.erase(g_graphs.begin());
     }
-    g_graphs.emplace_back(k, exec);
+    g_graphs.push_back(CachedGraph{k, exec, done});
+    e = launch_recorded(g_graphs.back(), stream);
+    return e != hipSuccess ? e : first;
 }
 
 static int graph_cache_clear() {
     std::lock_guard<std::mutex> lock(g_graph_mutex);
     hipError_t first = hipSuccess;
-    for (auto& e : g_graphs) {
-        const hipError_t r = hipGraphExecDestroy(e.second);
-        if (first == hipSuccess) first = r;
-    }
+    for (auto& g : g_graphs) destroy_cached(g, first);
+    for (auto& g : g_evicted) destroy_cached(g, first);
     g_graphs.clear();
+    g_evicted.clear();
     if (first != hipSuccess) return hip_status(first, "graph release");
     return CBCT_OK;
 }
@@ -473,27 +518,29 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
     key.total_params = total_params; key.max_cap = max_cap; key.grid = grid; key.big = big ? 1 : 0;
     for (int k = 0; k < 8; ++k) key.lr[k] = lr8 ? lr8[k] : 0.f;
     key.lr_decay = lr_decay; key.beta1 = beta1; key.beta2 = beta2; key.eps = eps; key.lambda0 = lambda0;
-    hipGraphExec_t exec = graph_cache_find(key);
-    if (exec == nullptr) {
-        // capture on a private stream (the caller's may be the null stream, which cannot be captured)
-        hipStream_t cs = nullptr;
-        CBCT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
-        hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) { (void)hipStreamDestroy(cs); return hip_status(e, "graph capture begin"); }
-        int st = CBCT_OK;
-        for (int s = 0; s < nsteps && st == CBCT_OK; ++s) st = enqueue(s, true, cs);
-        if (st == CBCT_OK) st = enqueue(nsteps, false, cs);
-        e = hipStreamEndCapture(cs, &graph);
-        (void)hipStreamDestroy(cs);
-        if (st != CBCT_OK) { if (graph) (void)hipGraphDestroy(graph); return st; }
-        if (e != hipSuccess) return hip_status(e, "graph capture end");
-        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) return hip_status(e, "graph instantiate");
-        graph_cache_insert(key, exec);
+    hipError_t launched = hipSuccess;
+    if (graph_cache_launch(key, stream, launched)) {
+        CBCT_HIP(launched, "graph launch");
+        return CBCT_OK;
     }
-    CBCT_HIP(hipGraphLaunch(exec, stream), "graph launch");
+    // capture on a private stream (the caller's may be the null stream, which cannot be captured)
+    hipStream_t cs = nullptr;
+    CBCT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) { (void)hipStreamDestroy(cs); return hip_status(e, "graph capture begin"); }
+    int st = CBCT_OK;
+    for (int s = 0; s < nsteps && st == CBCT_OK; ++s) st = enqueue(s, true, cs);
+    if (st == CBCT_OK) st = enqueue(nsteps, false, cs);
+    e = hipStreamEndCapture(cs, &graph);
+    (void)hipStreamDestroy(cs);
+    if (st != CBCT_OK) { if (graph) (void)hipGraphDestroy(graph); return st; }
+    if (e != hipSuccess) return hip_status(e, "graph capture end");
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_status(e, "graph instantiate");
+    CBCT_HIP(graph_cache_insert_launch(key, exec, stream), "graph launch");
     return CBCT_OK;
 }
 

@@ -342,8 +342,10 @@ int cbct_calib_run(const cbct_calib_problem* problems, int nproblems, int nsteps
  * adam_v may be NULL.  loss_history[i] = the accepted loss after i iterations;
  * the geometry ends at the accepted iterate.  No counterpart in the reference
  * (its calibration loop is left to user code). */
-/* Frees the instantiated graphs cached by cbct_calib_run / cbct_calib_run_lm (graphs still
- * running are freed when they complete).  No reference counterpart. */
+/* Frees the instantiated graphs cached by cbct_calib_run / cbct_calib_run_lm (the 8 most recent)
+ * and those evicted from the cache whose last launch has not yet been seen to complete (the others
+ * are freed by later calls without waiting).  Call it when no calibration launched by the library
+ * is still running (after synchronising the streams that ran them).  No reference counterpart. */
 int cbct_calib_release_graphs(void);
 
 size_t cbct_calib_lm_workspace_bytes(const cbct_calib_problem* problems, int nproblems);

@@ -214,10 +214,9 @@ __global__ __launch_bounds__(FP_THREADS) void fp_count_kernel(
 // one for the adjoint that also scatters the volume gradient into LDS: gathers favour 8x8 lane
 // blocks (neighbouring taps share banks and cache lines), the LDS integer atomics favour the
 // interleaved layout (fewer lanes of a wave on one bank).  Defaults measured on the 256^3 x 720
-// views, 768x640 workload; their LDS capacities are the largest round sizes that keep the
-// occupancy the registers allow (8 workgroups of one 20 KB box per CU for the gathering kernels;
-// 5 workgroups of two 15 KB boxes plus 160 B of reductions for the adjoint -- two 16 KB boxes
-// would leave room for only 4).
+// views, 768x640 workload: the gathering kernels' 20 KB box leaves room for 7 workgroups per CU (a
+// box that fits 8 measured the same); the adjoint's two 15 KB boxes plus its reductions keep the 5
+// workgroups its registers allow (two 16 KB boxes would leave room for only 4).
 struct FpTuning {
     int use_slab;         // slab-staged kernels (1) or direct kernels (0)
     int T;                // slab thickness in voxel planes

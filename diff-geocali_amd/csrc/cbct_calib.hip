@@ -450,6 +450,8 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
     }
     CalibDev* ddev = (CalibDev*)(ws + L.descs);
     unsigned* counter = (unsigned*)(ws + L.counter);
+    int device = 0, st0;
+    if ((st0 = current_stream_device(stream, &device))) return st0;   // graphs are captured on it
     // descriptors: written by kernels whose arguments carry them, ordered on the stream before
     // every step (no host synchronisation)
     for (int i0 = 0; i0 < nproblems; i0 += DESC_BATCH) {
@@ -459,8 +461,7 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
         calib_desc_kernel<<<1, 256, 0, stream>>>(batch, n, ddev + i0);
         CBCT_LAUNCHED("calibration descriptor upload");
     }
-    int device = 0, cus = 0;
-    CBCT_HIP(hipGetDevice(&device), "hipGetDevice");
+    int cus = 0;
     CBCT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
     const int grid = std::max(1, std::min(total_tiles, cus * 8));
     AdamArgs h;

@@ -360,6 +360,10 @@ int check_finite(float x, const char* name);
 // step_size against the FP32 resolution of the ray parameter over the volume: a step below
 // 2^-22 (volume diagonal + 1) leaves every view degenerate (guard_frame), so it is refused.
 int check_step_resolution(float step, const FrameGuard& G);
+// The device of the calling thread, after checking that the stream (if not the null stream) belongs
+// to it: the entry points that keep per-device state (the allocation-free adjoint's scratch, the
+// calibration's captured graphs) look that state up on the current device.
+int current_stream_device(hipStream_t stream, int* device);
 
 __global__ void frames_kernel(GeomPtrs g, int A, FrameGuard guard, ViewFrame* __restrict__ out);
 __global__ void fill_kernel(float* __restrict__ p, int64_t n, float value);

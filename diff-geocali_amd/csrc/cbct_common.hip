@@ -40,6 +40,20 @@ int check_finite(float x, const char* name) {
     return CBCT_OK;
 }
 
+int current_stream_device(hipStream_t stream, int* device) {
+    int cur = -1;
+    CBCT_HIP(hipGetDevice(&cur), "hipGetDevice");
+    if (stream != nullptr) {
+        hipDevice_t sd = -1;
+        CBCT_HIP(hipStreamGetDevice(stream, &sd), "hipStreamGetDevice");
+        if ((int)sd != cur)
+            return fail(CBCT_EINVAL, "the stream belongs to device " + std::to_string((int)sd) + " but device " +
+                                         std::to_string(cur) + " is current; make the stream's device current");
+    }
+    *device = cur;
+    return CBCT_OK;
+}
+
 int check_step_resolution(float step, const FrameGuard& G) {
     if ((double)step < (2.0 * (double)G.R + 1.0) * 0x1p-22)
         return fail(CBCT_EINVAL, "step_size is below the FP32 resolution of the ray parameter over the volume "

@@ -590,10 +590,11 @@ constexpr int MAX_DEVICES = 64;
 static std::mutex g_scratch_mutex;
 static void* g_scratch_ptr[MAX_DEVICES] = {};
 
-// The current device's address of the scratch array (looked up on the first call on the device).
-static int scratch_address(void** out) {
-    int dev = -1;
-    CBCT_HIP(hipGetDevice(&dev), "hipGetDevice");
+// The current device's address of the scratch array (looked up on the first call on the device);
+// the stream must belong to that device.
+static int scratch_address(hipStream_t stream, void** out) {
+    int dev = -1, st;
+    if ((st = current_stream_device(stream, &dev))) return st;
     CBCT_CHECK_ARG(dev >= 0 && dev < MAX_DEVICES, "device ordinal beyond the library's 64-device table");
     std::lock_guard<std::mutex> lock(g_scratch_mutex);
     if (g_scratch_ptr[dev] == nullptr) {
@@ -627,7 +628,7 @@ extern "C" int cbct_fp_backward_noalloc(const float* grad_projection, const floa
                                          std::to_string((size_t)CBCT_NOALLOC_SCRATCH_BYTES) +
                                          " (use cbct_fp_backward_ex with a workspace)");
     void* ws = nullptr;
-    if ((st = scratch_address(&ws))) return st;
+    if ((st = scratch_address((hipStream_t)stream_, &ws))) return st;
     const float* g[8] = {angles, sid, sod, det_u0, det_v0, src_x, src_y, src_z};
     return fp_backward_impl(grad_projection, volume, D, H, W, g, A, U, V, pixel_size, step_size, voxel_size,
                             compute_grad_volume, off_x, off_y, off_z, grad_volume, grad_geometry, true, options, ws,

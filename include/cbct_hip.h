@@ -146,7 +146,8 @@ int cbct_fp_backward_ex(const float* grad_projection, const float* volume,
  * until the process ends), otherwise the call returns CBCT_EWORKSPACE without touching the
  * device.  It calls no allocating routine.  All calls on one GPU share that GPU's scratch, so
  * calls that overlap in time on one GPU (other streams, other host threads) race: order them.
- * Every byte a call reads from the scratch is written by the same call first. */
+ * Every byte a call reads from the scratch is written by the same call first.  The scratch used is
+ * the current device's: a stream of another device is refused with CBCT_EINVAL. */
 #define CBCT_NOALLOC_SCRATCH_BYTES (64ull << 20)
 int cbct_fp_backward_noalloc(const float* grad_projection, const float* volume,
                              int64_t D, int64_t H, int64_t W,
@@ -328,7 +329,9 @@ size_t cbct_calib_workspace_bytes(const cbct_calib_problem* problems, int nprobl
  * so bitwise reproducible).  use_graph != 0 captures all steps in one hipGraph and launches
  * it; the instantiated graph is cached by the call's sizes, workspace address and optimiser
  * settings (up to 8 graphs; cbct_calib_release_graphs frees them), so repeated calls with the
- * same key launch it again without capturing.  No part of the call synchronises the stream. */
+ * same key launch it again without capturing.  No part of the call synchronises the stream.
+ * Graphs are captured on the current device: a stream of another device is refused with
+ * CBCT_EINVAL. */
 int cbct_calib_run(const cbct_calib_problem* problems, int nproblems, int nsteps, int first_step,
                    const float* lr8, float lr_decay, float beta1, float beta2, float eps, int use_graph,
                    void* workspace, size_t workspace_bytes, void* stream);

@@ -244,3 +244,23 @@ def test_fitting_only_the_detector_shifts_recovers_them():
     print(f"detector shift error {err0:.3f} mm -> {err1:.3f} mm; loss {hist[0].item():.2e} -> {hist[-1].item():.2e}")
     assert err1 < 0.2 * err0
     assert torch.equal(geo[0], p.geometry[0]) and torch.equal(geo[5], p.geometry[5])
+
+
+def test_graph_cache_eviction_reuse_and_release():
+    """More distinct graphs than the cache keeps (eight): keys captured again after their eviction
+    give the same fit bit for bit, the release frees the cached and the evicted graphs, and a call
+    after it captures afresh with the same result."""
+    runs = {}
+    for steps in list(range(1, 11)) + [1, 2]:
+        p = _problem(SMALL)
+        calibration.calibrate([p], steps=steps, use_graph=True)
+        runs.setdefault(steps, []).append((p.geometry.clone(), p.loss_history.clone()))
+    for steps in (1, 2):
+        (g0, l0), (g1, l1) = runs[steps]
+        assert torch.equal(g0, g1) and torch.equal(l0, l1)
+    torch.cuda.synchronize()
+    assert _native.lib().cbct_calib_release_graphs() == 0
+    p = _problem(SMALL)
+    calibration.calibrate([p], steps=3, use_graph=True)
+    g3, l3 = runs[3][0]
+    assert torch.equal(p.geometry, g3) and torch.equal(p.loss_history, l3)

@@ -555,44 +555,28 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             };
             if (staged) {
                 if (!BCAST) __syncthreads();
-                // four loads in flight per lane before their LDS stores
-                constexpr int NB = 4;
-                if (walk) {
-                    walk_start();
-                    for (int i0 = threadIdx.x; i0 < nbox; i0 += NB * SL_THREADS) {
-                        float vv[NB];
-#pragma unroll
-                        for (int k = 0; k < NB; ++k) {
-                            const int i = i0 + k * SL_THREADS;
-                            vv[k] = TAPS && i < nbox ? P.vol[(VoxelIndex)woff] : 0.f;
-                            walk_next();
-                        }
-#pragma unroll
-                        for (int k = 0; k < NB; ++k) {
-                            const int i = i0 + k * SL_THREADS;
-                            if (i < nbox) {
-                                if (TAPS) vbox[i] = vv[k];
-                                if (MODE == 2) gboxu[i] = 0u;
-                            }
-                        }
+                // Box element i goes to vbox[i]; wave w stages elements c + 64 w + lane of each
+                // chunk c = 0, 256, ...  A whole 64-element run is loaded straight into LDS
+                // (global_load_lds_dword: lane l's dword lands at the wave-uniform LDS address +
+                // 4 l), so every load of the box is in flight at once and holds no register; a
+                // partial last run goes through registers.  The gradient box is zeroed alongside.
+                const int wbase = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
+                if (walk) walk_start();
+                for (int i0 = wbase; i0 < nbox; i0 += SL_THREADS) {
+                    const int i = i0 + lane;
+                    if (TAPS) {
+                        const VoxelIndex vi = walk ? (VoxelIndex)woff : box_voxel(i);
+                        if (i0 + 64 <= nbox)
+                            __builtin_amdgcn_global_load_lds(
+                                (const void*)(P.vol + vi), (__attribute__((address_space(3))) void*)(vbox + i0), 4, 0, 0);
+                        else if (i < nbox)
+                            vbox[i] = P.vol[vi];
                     }
-                } else
-                for (int i0 = threadIdx.x; i0 < nbox; i0 += NB * SL_THREADS) {
-                    float vv[NB];
-#pragma unroll
-                    for (int k = 0; k < NB; ++k) {
-                        const int i = i0 + k * SL_THREADS;
-                        vv[k] = TAPS && i < nbox ? P.vol[box_voxel(i)] : 0.f;
-                    }
-#pragma unroll
-                    for (int k = 0; k < NB; ++k) {
-                        const int i = i0 + k * SL_THREADS;
-                        if (i < nbox) {
-                            if (TAPS) vbox[i] = vv[k];
-                            if (MODE == 2) gboxu[i] = 0u;
-                        }
-                    }
+                    if (MODE == 2 && i < nbox) gboxu[i] = 0u;
+                    if (walk) walk_next();
                 }
+                // s_waitcnt vmcnt(0): the direct-to-LDS loads have landed before the barrier
+                if (TAPS) __builtin_amdgcn_s_waitcnt(0x0F70);
                 __syncthreads();
             }
             // advance the lane's ray through this slab (principal axis and direction as constants)

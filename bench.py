@@ -76,20 +76,20 @@ DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 # WRITE_SIZE x 1.0 (KiB x 1024).  The counts depend only on the workload, the kernel and its tuning
 # (SQ_WAIT_INST_LDS and SQ_WAVE_CYCLES vary by about 1% between runs).
 MEASURED = {
-    ("geometry-gradients-256c-720v", "adjoint"): dict(valu=112_309_772_582, lds_conflict=38_899_405_936,
-                                                      lds_active=74_130_204_968, lds_wait=53_237_905_916,
-                                                      wave_qcycles=456_166_095_225, grbm=3_038_617_749,
-                                                      fetch_kib=105_302_061, write_kib=138_735_632),
-    ("geometry-gradients-256c-720v", "forward"): dict(valu=54_536_859_549, lds_conflict=9_670_306_160,
-                                                      lds_active=27_012_445_724, lds_wait=12_673_369_010,
-                                                      wave_qcycles=270_325_842_926, grbm=1_283_814_838,
-                                                      fetch_kib=91_412_406, write_kib=1_465_855),
+    ("geometry-gradients-256c-720v", "adjoint"): dict(valu=113_357_391_648, lds_conflict=38_899_405_936,
+                                                      lds_active=73_355_711_832, lds_wait=49_859_088_744,
+                                                      wave_qcycles=453_988_065_090, grbm=3_026_922_843,
+                                                      fetch_kib=100_220_168, write_kib=139_240_895),
+    ("geometry-gradients-256c-720v", "forward"): dict(valu=53_543_571_102, lds_conflict=9_670_306_160,
+                                                      lds_active=26_231_704_402, lds_wait=11_725_480_890,
+                                                      wave_qcycles=259_789_671_254, grbm=1_248_358_994,
+                                                      fetch_kib=90_301_252, write_kib=1_474_642),
     # one fused Adam march of the 64-problem batched calibration (all problems, one launch)
-    (CALIB_WORKLOAD, "march"): dict(valu=1_274_549_605_032, lds_conflict=196_736_971_792, grbm=30_497_691_569,
-                                    fetch_kib=3_232_548_829, write_kib=58_107_987),
+    (CALIB_WORKLOAD, "march"): dict(valu=1_255_542_653_659, lds_conflict=179_891_963_696, grbm=29_455_398_767,
+                                    fetch_kib=3_137_745_506, write_kib=86_052_683),
     # forward at 360 views, 512x384 (FDK round trip); backprojection backward / forward at 256^3 x 360
     # views (HBM counters only)
-    ("fdk-roundtrip-256c-360v", "forward"): dict(fetch_kib=47_640_337, write_kib=296_104),
+    ("fdk-roundtrip-256c-360v", "forward"): dict(fetch_kib=47_404_480, write_kib=296_951),
     ("backprojection-256c-360v", "bp_backward"): dict(fetch_kib=1_595_804, write_kib=6_993_152),
     ("backprojection-256c-360v", "bp_forward"): dict(fetch_kib=8_328_817, write_kib=172_207),
 }

@@ -103,7 +103,12 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_forward_kernel(
 // eight per-view geometry derivatives (csrc/src_cbct_backprojector_cuda.cu:101-295).
 //
 // One workgroup owns an 8x8 block of voxel columns and 64*K voxels of each along z; each wave
-// takes 16 of the block's columns (lane = z).  Per view the block's taps fall in a small
+// takes 16 of the block's columns (lane = z).  Per view the z-independent factors of a column
+// (its detector row and weight, and the coefficients that turn its four per-voxel sums into the
+// eight geometry terms) are computed once, by lane L for the block's column L, and handed to
+// the wave that takes that column through LDS (16-byte broadcast reads), instead of by all 64
+// lanes of that wave for each of its 16 columns: 133 -> 88 VALU instructions per voxel-view,
+// 18.1 -> 15.4 ms at 256^3 x 360.  Per view the block's taps fall in a small
 // detector window (columns of the block share a few detector rows u, and a column covers a run
 // of v): the workgroup bounds that window exactly (same float expressions as the taps), clears
 // an LDS tile for it, adds its projection-gradient contributions there as 32-bit fixed-point
@@ -123,6 +128,7 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_forward_kernel(
 constexpr int BT = 8;                          // column block edge
 constexpr int BT_COLS_PER_WAVE = BT * BT / BP_WAVES;
 constexpr int BT_CAP = 8192;                   // LDS tile cells
+constexpr int BP_COLF = 24;                    // per-column factor floats (22 used)
 
 struct BpColumn {
     float px, py, m, q, l, iq, iq2, iq3, t, du, gw, ui;
@@ -163,6 +169,8 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
     __shared__ unsigned tile[BT_CAP];
     __shared__ float red[BP_WAVES];
     __shared__ float gred[BP_WAVES][8];
+    // per view, the z-independent factors of each of the block's 64 columns (BP_COLF floats each)
+    __shared__ float4 colf[BT * BT * BP_COLF / 4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x, nblk = gridDim.x * gridDim.y;
     const int nbx = (g.W + BT - 1) / BT;
@@ -204,8 +212,9 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
         const ViewFrame f = frames[a];
         // ---- exact detector window of the block's taps (uniform over the workgroup)
         float umin = INF, umax = -INF, vmin = INF, vmax = -INF, gwmax = 0.f, tmin = INF;
+        BpColumn c{};
         if (bcol) {
-            const BpColumn c = bp_column(f, bvx, bvy, ip);
+            c = bp_column(f, bvx, bvy, ip);
             if (c.ok) {
                 const float v1 = bp_vi(f, c.t, vz_lo, ip), v2 = bp_vi(f, c.t, vz_hi, ip);
                 umin = umax = c.ui;
@@ -248,45 +257,70 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
             inv_scale = ldexpf(1.f, -e);
             for (int i = threadIdx.x; i < nu * nv; i += 64 * BP_WAVES) tile[i] = 0u;
         }
+        // The z-independent factors of lane L's column (the block's column L), once per view, for
+        // the wave that takes that column (L mod BP_WAVES): the same expressions as the per-column
+        // code computed them in every lane of that wave for every column.
+        if ((lane & (BP_WAVES - 1)) == wave) {
+            const bool cvalid = bcol && c.ok && c.ui >= 0.0f && c.ui < Um1;
+            float cf[BP_COLF];
+#pragma unroll
+            for (int i = 0; i < BP_COLF; ++i) cf[i] = 0.f;
+            if (cvalid) {
+                const float iq = c.iq, iq2 = c.iq2, iq3 = c.iq3, t = c.t, du = c.du;
+                const float m = c.m, l = c.l, q = c.q, px = c.px, py = c.py;
+                const float m2 = m * m;
+                const int iu = (int)floorf(c.ui);
+                const float fu = c.ui - (float)iu;
+                const float dt_sod = -l * iq2;
+                const float dW_m = 2.0f * m * iq2 - 2.0f * m2 * iq3;     // d(m^2/q^2)/dm at fixed q-m
+                const float dsu_b = f.dsn, dsn_b = -f.dsu;
+                const float dq_b = dsn_b + px;
+                const float dt_b = (dsn_b * q - l * dq_b) * iq2;
+                const float dU_b = fmaf(du, dt_b, fmaf(1.0f - t, dsu_b, t * py));
+                const float dW_b = 2.0f * m * iq2 * dsn_b - 2.0f * m2 * iq3 * dq_b;
+                const float qml = q - l;
+                const float dt_sx = -f.s * qml * iq2;
+                const float dU_sx = fmaf(du, dt_sx, (1.0f - t) * f.c);
+                const float dW_sx = dW_m * -f.s;
+                const float dt_sy = f.c * qml * iq2;
+                const float dU_sy = fmaf(du, dt_sy, (1.0f - t) * f.s);
+                const float dW_sy = dW_m * f.c;
+                cf[0] = __int_as_float(1); cf[1] = __int_as_float(iu); cf[2] = fu; cf[3] = 1.0f - fu;
+                cf[4] = t; cf[5] = c.gw; cf[6] = __int_as_float((iu - u_lo) * nv);
+                cf[7] = iq * ip; cf[8] = du * iq * ip;                        // sid
+                cf[9] = dt_sod * ip; cf[10] = du * dt_sod * ip; cf[11] = dW_m;  // sod
+                cf[12] = dt_b * ip; cf[13] = dU_b * ip; cf[14] = dW_b;        // beta
+                cf[15] = dt_sx * ip; cf[16] = dU_sx * ip; cf[17] = dW_sx;     // source x
+                cf[18] = dt_sy * ip; cf[19] = dU_sy * ip; cf[20] = dW_sy;     // source y
+                cf[21] = (1.0f - t) * ip;                                     // source z
+            }
+#pragma unroll
+            for (int i = 0; i < BP_COLF / 4; ++i)
+                colf[lane * (BP_COLF / 4) + i] = make_float4(cf[4 * i], cf[4 * i + 1], cf[4 * i + 2], cf[4 * i + 3]);
+        }
         __syncthreads();
         float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f;
 #pragma unroll 1
         for (int j = 0; j < BT_COLS_PER_WAVE; ++j) {
             const int cidx = wave + BP_WAVES * j;
-            const int x = bx * BT + cidx % BT, y = by * BT + cidx / BT;
-            if (x >= g.W || y >= g.H) continue;
-            const float vx = fmaf((float)x - g.hx, g.vs, g.offx);
-            const float vy = fmaf((float)y - g.hy, g.vs, g.offy);
-            const BpColumn c = bp_column(f, vx, vy, ip);
-            if (!c.ok) continue;
-            if (!(c.ui >= 0.0f && c.ui < Um1)) continue;
-            const float iq = c.iq, iq2 = c.iq2, iq3 = c.iq3, t = c.t, du = c.du, gw = c.gw;
-            const float m = c.m, l = c.l, q = c.q, px = c.px, py = c.py;
-            const float m2 = m * m;
-            const int iu = (int)floorf(c.ui);
-            const float fu = c.ui - (float)iu, mu = 1.0f - fu;
-            const int64_t base = ((int64_t)a * g.U + iu) * g.V;
-            // z-independent derivative factors
-            const float dt_sod = -l * iq2;
-            const float dW_m = 2.0f * m * iq2 - 2.0f * m2 * iq3;     // d(m^2/q^2)/dm at fixed q-m
-            const float dsu_b = f.dsn, dsn_b = -f.dsu;
-            const float dq_b = dsn_b + px;
-            const float dt_b = (dsn_b * q - l * dq_b) * iq2;
-            const float dU_b = fmaf(du, dt_b, fmaf(1.0f - t, dsu_b, t * py));
-            const float dW_b = 2.0f * m * iq2 * dsn_b - 2.0f * m2 * iq3 * dq_b;
-            const float qml = q - l;
-            const float dt_sx = -f.s * qml * iq2;
-            const float dU_sx = fmaf(du, dt_sx, (1.0f - t) * f.c);
-            const float dW_sx = dW_m * -f.s;
-            const float dt_sy = f.c * qml * iq2;
-            const float dU_sy = fmaf(du, dt_sy, (1.0f - t) * f.s);
-            const float dW_sy = dW_m * f.c;
+            // (all six 16-byte reads issued before the column's validity is looked at)
+            float4 c4[BP_COLF / 4];
+#pragma unroll
+            for (int i = 0; i < BP_COLF / 4; ++i) c4[i] = colf[cidx * (BP_COLF / 4) + i];
+            const float cf[BP_COLF] = {c4[0].x, c4[0].y, c4[0].z, c4[0].w, c4[1].x, c4[1].y, c4[1].z, c4[1].w,
+                                       c4[2].x, c4[2].y, c4[2].z, c4[2].w, c4[3].x, c4[3].y, c4[3].z, c4[3].w,
+                                       c4[4].x, c4[4].y, c4[4].z, c4[4].w, c4[5].x, c4[5].y, c4[5].z, c4[5].w};
+            if (__float_as_int(cf[0]) == 0) continue;   // column outside the volume, behind the source or off the detector
+            const int iu = __float_as_int(cf[1]);
+            const float fu = cf[2], mu = cf[3], t = cf[4], gw = cf[5];
+            const int row = __float_as_int(cf[6]);
+            // the tap offsets fit in 32 bits: A * U * V <= INT32_MAX (host check)
+            const int base = (a * g.U + iu) * g.V;
             // Every geometry term of a voxel is a combination, with coefficients constant along the
             // column, of four per-voxel products: gb*Iv*dv, gb*Iu, gb*Iv and g*interp.  The column
             // sums those four and combines them once (the same terms as :214-292, summed in
             // another order).
             float P1 = 0.f, P2 = 0.f, P3 = 0.f, P4 = 0.f;
-            const int row = (iu - u_lo) * nv;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const float dv = vz[k] - f.dsv;
@@ -295,7 +329,7 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
                 if (z0 + lane + 64 * k < g.D && vi >= 0.0f && vi < Vm1) {
                     const int iv = (int)floorf(vi);
                     const float fv = vi - (float)iv, mv = 1.0f - fv;
-                    const int64_t k00 = base + iv;
+                    const int k00 = base + iv;
                     const float p00 = proj[k00], p01 = proj[k00 + 1];
                     const float p10 = proj[k00 + g.V], p11 = proj[k00 + g.V + 1];
                     const float interp = fmaf(fu * fv, p11, fmaf(mu * fv, p01, fmaf(fu * mv, p10, mu * mv * p00)));
@@ -303,11 +337,13 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
                     const float Iv = fmaf(fu, p11 - p10, mu * (p01 - p00));
                     const float gb = gv[j][k] * gw;
                     if (staged) {
+                        // scale is a power of two, so (gb scale) w1 w2 rounds exactly as (gb w1 w2) scale
+                        const float gbs = gb * scale, gm = gbs * mu, gf = gbs * fu;
                         unsigned* q0 = tile + row + (iv - v_lo);
-                        atomicAdd(q0, (unsigned)__float2int_rn(gb * mu * mv * scale));
-                        atomicAdd(q0 + nv, (unsigned)__float2int_rn(gb * fu * mv * scale));
-                        atomicAdd(q0 + 1, (unsigned)__float2int_rn(gb * mu * fv * scale));
-                        atomicAdd(q0 + nv + 1, (unsigned)__float2int_rn(gb * fu * fv * scale));
+                        atomicAdd(q0, (unsigned)__float2int_rn(gm * mv));
+                        atomicAdd(q0 + nv, (unsigned)__float2int_rn(gf * mv));
+                        atomicAdd(q0 + 1, (unsigned)__float2int_rn(gm * fv));
+                        atomicAdd(q0 + nv + 1, (unsigned)__float2int_rn(gf * fv));
                     } else if (DET) {
                         atomicAdd(gacc + k00, (unsigned long long)__double2ll_rn(ldexp((double)(gb * mu * mv), Ea)));
                         atomicAdd(gacc + k00 + g.V, (unsigned long long)__double2ll_rn(ldexp((double)(gb * fu * mv), Ea)));
@@ -328,18 +364,18 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
                 }
             }
             // sid (:214-219): gb (Iv dv + Iu du) / q / pix
-            s1 += fmaf(P1, iq * ip, P2 * (du * iq * ip));
+            s1 += fmaf(P1, cf[7], P2 * cf[8]);
             // sod (:222-230)
-            s2 += fmaf(P1, dt_sod * ip, fmaf(P2, du * dt_sod * ip, P4 * dW_m));
+            s2 += fmaf(P1, cf[9], fmaf(P2, cf[10], P4 * cf[11]));
             // beta (:233-250)
-            s0 += fmaf(P1, dt_b * ip, fmaf(P2, dU_b * ip, P4 * dW_b));
+            s0 += fmaf(P1, cf[12], fmaf(P2, cf[13], P4 * cf[14]));
             // detector centres (:252-253)
             s3 += P2;
             s4 += P3;
             // source offsets (:255-292)
-            s5 += fmaf(P1, dt_sx * ip, fmaf(P2, dU_sx * ip, P4 * dW_sx));
-            s6 += fmaf(P1, dt_sy * ip, fmaf(P2, dU_sy * ip, P4 * dW_sy));
-            s7 += P3 * ((1.0f - t) * ip);
+            s5 += fmaf(P1, cf[15], fmaf(P2, cf[16], P4 * cf[17]));
+            s6 += fmaf(P1, cf[18], fmaf(P2, cf[19], P4 * cf[20]));
+            s7 += P3 * cf[21];
         }
         s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
         s4 = wave_sum(s4); s5 = wave_sum(s5); s6 = wave_sum(s6); s7 = wave_sum(s7);

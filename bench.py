@@ -87,11 +87,16 @@ MEASURED = {
     # one fused Adam march of the 64-problem batched calibration (all problems, one launch)
     (CALIB_WORKLOAD, "march"): dict(valu=1_255_542_653_659, lds_conflict=179_891_963_696, grbm=29_455_398_767,
                                     fetch_kib=3_137_745_506, write_kib=86_052_683),
-    # forward at 360 views, 512x384 (FDK round trip); backprojection backward / forward at 256^3 x 360
-    # views (HBM counters only)
+    # forward at 360 views, 512x384 (FDK round trip; HBM counters only); backprojection backward /
+    # forward at 256^3 x 360 views
     ("fdk-roundtrip-256c-360v", "forward"): dict(fetch_kib=47_404_480, write_kib=296_951),
-    ("backprojection-256c-360v", "bp_backward"): dict(fetch_kib=1_595_804, write_kib=6_993_152),
-    ("backprojection-256c-360v", "bp_forward"): dict(fetch_kib=8_328_817, write_kib=172_207),
+    ("backprojection-256c-360v", "bp_backward"): dict(valu=8_317_779_805, lds_conflict=896_532_168,
+                                                      lds_active=5_022_414_146, lds_wait=3_695_226_014,
+                                                      wave_qcycles=31_317_439_506, grbm=290_898_786,
+                                                      fetch_kib=1_641_907, write_kib=6_993_152),
+    ("backprojection-256c-360v", "bp_forward"): dict(valu=3_143_892_992, lds_conflict=0, lds_active=0, lds_wait=0,
+                                                     wave_qcycles=25_888_977_865, grbm=111_409_064,
+                                                     fetch_kib=8_304_061, write_kib=172_206),
 }
 CUS, XCDS = 256, 8
 

@@ -160,6 +160,35 @@ __device__ __forceinline__ float bp_vi(const ViewFrame& f, float t, float vz, fl
     return fmaf(vph, ip, f.v0);
 }
 
+// Wave-wide (64 lanes) reductions through DPP row operations: VALU only, where the __shfl_xor
+// butterfly costs an LDS permute per step.  The two quad permutations, the half-row and the row
+// mirror leave every lane of a row of 16 with that row's total; the row broadcasts of lanes 15 and
+// 31 carry rows 0..2 into row 3, and lane 63's value is returned as a wave-uniform scalar.  Every
+// lane of the wave must be active.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ float dpp_from(float v, float ident) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(ident), __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+template <class Op>
+__device__ __forceinline__ float wave_reduce(float v, float ident, Op op) {
+    v = op(v, dpp_from<0xB1>(v, ident));         // quad_perm [1,0,3,2]
+    v = op(v, dpp_from<0x4E>(v, ident));         // quad_perm [2,3,0,1]
+    v = op(v, dpp_from<0x141>(v, ident));        // row_half_mirror
+    v = op(v, dpp_from<0x140>(v, ident));        // row_mirror
+    v = op(v, dpp_from<0x142, 0xA>(v, ident));   // row_bcast:15 into rows 1 and 3
+    v = op(v, dpp_from<0x143, 0xC>(v, ident));   // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ float wave_min(float v) {
+    return wave_reduce(v, __builtin_huge_valf(), [](float a, float b) { return fminf(a, b); });
+}
+__device__ __forceinline__ float wave_max(float v) {
+    return wave_reduce(v, -__builtin_huge_valf(), [](float a, float b) { return fmaxf(a, b); });
+}
+__device__ __forceinline__ float wave_add(float v) {
+    return wave_reduce(v, 0.0f, [](float a, float b) { return a + b; });
+}
+
 template <int K, bool DET>
 __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
     const float* __restrict__ gvol, const float* __restrict__ proj,
@@ -224,14 +253,9 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
                 tmin = c.t;
             }
         }
-        for (int s = 32; s >= 1; s >>= 1) {
-            umin = fminf(umin, __shfl_xor(umin, s, 64));
-            umax = fmaxf(umax, __shfl_xor(umax, s, 64));
-            vmin = fminf(vmin, __shfl_xor(vmin, s, 64));
-            vmax = fmaxf(vmax, __shfl_xor(vmax, s, 64));
-            gwmax = fmaxf(gwmax, __shfl_xor(gwmax, s, 64));
-            tmin = fminf(tmin, __shfl_xor(tmin, s, 64));
-        }
+        umin = wave_min(umin); umax = wave_max(umax);
+        vmin = wave_min(vmin); vmax = wave_max(vmax);
+        gwmax = wave_max(gwmax); tmin = wave_min(tmin);
         float* part = partials + ((int64_t)a * nblk + blk) * 8;
         if (!(umax >= 0.f && umin < Um1 && vmax >= 0.f && vmin < Vm1)) {      // no tap on the detector
             if (threadIdx.x < 8) part[threadIdx.x] = 0.f;
@@ -377,8 +401,8 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
             s6 += fmaf(P1, cf[18], fmaf(P2, cf[19], P4 * cf[20]));
             s7 += P3 * cf[21];
         }
-        s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
-        s4 = wave_sum(s4); s5 = wave_sum(s5); s6 = wave_sum(s6); s7 = wave_sum(s7);
+        s0 = wave_add(s0); s1 = wave_add(s1); s2 = wave_add(s2); s3 = wave_add(s3);
+        s4 = wave_add(s4); s5 = wave_add(s5); s6 = wave_add(s6); s7 = wave_add(s7);
         if (lane == 0) {
             gred[wave][0] = s0; gred[wave][1] = s1; gred[wave][2] = s2; gred[wave][3] = s3;
             gred[wave][4] = s4; gred[wave][5] = s5; gred[wave][6] = s6; gred[wave][7] = s7;

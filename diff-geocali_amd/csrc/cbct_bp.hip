@@ -327,10 +327,13 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
 #pragma unroll 1
         for (int j = 0; j < BT_COLS_PER_WAVE; ++j) {
             const int cidx = wave + BP_WAVES * j;
-            // (all six 16-byte reads issued before the column's validity is looked at)
-            float4 c4[BP_COLF / 4];
+            // (six 16-byte reads, all issued before the column's validity is looked at: volatile
+            // keeps each one whole)
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            f4 c4[BP_COLF / 4];
 #pragma unroll
-            for (int i = 0; i < BP_COLF / 4; ++i) c4[i] = colf[cidx * (BP_COLF / 4) + i];
+            for (int i = 0; i < BP_COLF / 4; ++i)
+                c4[i] = *(const volatile __attribute__((address_space(3))) f4*)(colf + cidx * (BP_COLF / 4) + i);
             const float cf[BP_COLF] = {c4[0].x, c4[0].y, c4[0].z, c4[0].w, c4[1].x, c4[1].y, c4[1].z, c4[1].w,
                                        c4[2].x, c4[2].y, c4[2].z, c4[2].w, c4[3].x, c4[3].y, c4[3].z, c4[3].w,
                                        c4[4].x, c4[4].y, c4[4].z, c4[4].w, c4[5].x, c4[5].y, c4[5].z, c4[5].w};

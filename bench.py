@@ -90,13 +90,13 @@ MEASURED = {
     # forward at 360 views, 512x384 (FDK round trip; HBM counters only); backprojection backward /
     # forward at 256^3 x 360 views
     ("fdk-roundtrip-256c-360v", "forward"): dict(fetch_kib=47_404_480, write_kib=296_951),
-    ("backprojection-256c-360v", "bp_backward"): dict(valu=8_317_779_805, lds_conflict=896_532_168,
-                                                      lds_active=5_022_414_146, lds_wait=3_695_226_014,
-                                                      wave_qcycles=31_317_439_506, grbm=290_898_786,
-                                                      fetch_kib=1_641_907, write_kib=6_993_152),
+    ("backprojection-256c-360v", "bp_backward"): dict(valu=8_566_132_836, lds_conflict=896_532_168,
+                                                      lds_active=3_087_791_426, lds_wait=818_363_786,
+                                                      wave_qcycles=28_923_119_949, grbm=268_885_770,
+                                                      fetch_kib=1_683_878, write_kib=6_993_152),
     ("backprojection-256c-360v", "bp_forward"): dict(valu=3_143_892_992, lds_conflict=0, lds_active=0, lds_wait=0,
-                                                     wave_qcycles=25_888_977_865, grbm=111_409_064,
-                                                     fetch_kib=8_304_061, write_kib=172_206),
+                                                     wave_qcycles=25_756_333_071, grbm=110_697_725,
+                                                     fetch_kib=8_417_371, write_kib=173_073),
 }
 CUS, XCDS = 256, 8
 

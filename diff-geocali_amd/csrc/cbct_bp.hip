@@ -76,8 +76,9 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_forward_kernel(
             const float vph = fmaf(t, vz[k] - f.dsv, f.dsv);
             const float vi = fmaf(vph, g.inv_pix, f.v0);
             if (vi >= 0.0f && vi < Vm1) {
-                const int iv = (int)floorf(vi);
-                const float fv = vi - (float)iv, mv = 1.0f - fv;
+                // vi >= 0: truncation is the floor, and the fraction is exact
+                const int iv = (int)vi;
+                const float fv = __builtin_amdgcn_fractf(vi), mv = 1.0f - fv;
                 const float p00 = row0[iv], p01 = row0[iv + 1], p10 = row1[iv], p11 = row1[iv + 1];
                 const float interp = fmaf(fu * fv, p11, fmaf(mu * fv, p01, fmaf(fu * mv, p10, mu * mv * p00)));
                 acc[k] = fmaf(gw, interp, acc[k]);
@@ -354,8 +355,9 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
                 const float vph = fmaf(t, dv, f.dsv);
                 const float vi = fmaf(vph, ip, f.v0);
                 if (z0 + lane + 64 * k < g.D && vi >= 0.0f && vi < Vm1) {
-                    const int iv = (int)floorf(vi);
-                    const float fv = vi - (float)iv, mv = 1.0f - fv;
+                    // vi >= 0: truncation is the floor, and the fraction is exact
+                    const int iv = (int)vi;
+                    const float fv = __builtin_amdgcn_fractf(vi), mv = 1.0f - fv;
                     const int k00 = base + iv;
                     const float p00 = proj[k00], p01 = proj[k00 + 1];
                     const float p10 = proj[k00 + g.V], p11 = proj[k00 + g.V + 1];

@@ -643,7 +643,7 @@ def preprocess_bench(args, torch, device):
         "config": {"workload": PREP_WORKLOAD, "views": A, "detector": [U, V], "layout_in": "AVU",
                    "ms_per_pass": t * 1e3},
         # algorithmic bytes (6 per pixel): the kernel reads and writes each byte once
-        "roofline": {"bound": "hbm", "kernel": "preprocess_kernel<uint16_t,1>", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "preprocess_views_kernel<uint16_t,8>", "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
     }), flush=True)
 

@@ -417,22 +417,23 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
             const int k = threadIdx.x;
             part[k] = (gred[0][k] + gred[1][k]) + (gred[2][k] + gred[3][k]);
         }
-        if (staged && DET) {
-            const int sh = Ea - e;      // cell units 2^-e -> accumulator units 2^-Ea
-            for (int r = 0; r < nu; ++r) {
-                unsigned long long* dst = gacc + ((int64_t)a * g.U + u_lo + r) * g.V + v_lo;
-                for (int cc = threadIdx.x; cc < nv; cc += 64 * BP_WAVES) {
-                    const int iv = (int)tile[r * nv + cc];
-                    if (iv != 0) atomicAdd(dst + cc, (unsigned long long)__double2ll_rn(ldexp((double)iv, sh)));
+        if (staged) {
+            // the tile's cells i = r * nv + c, r < nu, walked with every lane busy: from i to
+            // i + 256 the column moves by 256 mod nv and the row by 256 / nv or one more
+            const int n = nu * nv, dq = (64 * BP_WAVES) / nv, dr = (64 * BP_WAVES) - dq * nv;
+            int r = threadIdx.x / nv, cc = threadIdx.x - r * nv;
+            const int64_t row0 = ((int64_t)a * g.U + u_lo) * g.V + v_lo;
+            const int sh = Ea - e;      // (DET) cell units 2^-e -> accumulator units 2^-Ea
+            for (int i = threadIdx.x; i < n; i += 64 * BP_WAVES) {
+                const int iv = (int)tile[i];
+                if (iv != 0) {
+                    const int64_t k = row0 + (int64_t)r * g.V + cc;
+                    if (DET) atomicAdd(gacc + k, (unsigned long long)__double2ll_rn(ldexp((double)iv, sh)));
+                    else atomicAdd(gproj + k, (float)iv * inv_scale);
                 }
-            }
-        } else if (staged) {
-            for (int r = 0; r < nu; ++r) {
-                float* dst = gproj + ((int64_t)a * g.U + u_lo + r) * g.V + v_lo;
-                for (int cc = threadIdx.x; cc < nv; cc += 64 * BP_WAVES) {
-                    const int iv = (int)tile[r * nv + cc];
-                    if (iv != 0) atomicAdd(dst + cc, (float)iv * inv_scale);
-                }
+                cc += dr;
+                r += dq;
+                if (cc >= nv) { cc -= nv; ++r; }
             }
         }
         __syncthreads();

@@ -76,10 +76,14 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_forward_kernel(
             const float vph = fmaf(t, vz[k] - f.dsv, f.dsv);
             const float vi = fmaf(vph, g.inv_pix, f.v0);
             if (vi >= 0.0f && vi < Vm1) {
-                // vi >= 0: truncation is the floor, and the fraction is exact
+                // vi >= 0: truncation is the floor, and the fraction is exact; an unsigned 32-bit
+                // byte offset lets the loads take the wave-uniform row address as their base
                 const int iv = (int)vi;
                 const float fv = __builtin_amdgcn_fractf(vi), mv = 1.0f - fv;
-                const float p00 = row0[iv], p01 = row0[iv + 1], p10 = row1[iv], p11 = row1[iv + 1];
+                const unsigned bo = 4u * (unsigned)iv;      // byte offset in the row (< 2^31)
+                const float* const q0 = reinterpret_cast<const float*>(reinterpret_cast<const char*>(row0) + bo);
+                const float* const q1 = reinterpret_cast<const float*>(reinterpret_cast<const char*>(row1) + bo);
+                const float p00 = q0[0], p01 = q0[1], p10 = q1[0], p11 = q1[1];
                 const float interp = fmaf(fu * fv, p11, fmaf(mu * fv, p01, fmaf(fu * mv, p10, mu * mv * p00)));
                 acc[k] = fmaf(gw, interp, acc[k]);
             }
@@ -342,8 +346,12 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
             const int iu = __float_as_int(cf[1]);
             const float fu = cf[2], mu = cf[3], t = cf[4], gw = cf[5];
             const int row = __float_as_int(cf[6]);
-            // the tap offsets fit in 32 bits: A * U * V <= INT32_MAX (host check)
-            const int base = (a * g.U + iu) * g.V;
+            // the tap offsets fit in 32 bits: A * U * V <= INT32_MAX (host check); the column's
+            // row is wave-uniform, so its address goes to scalar registers and each tap load
+            // adds an unsigned 32-bit lane offset to it
+            const int base = __builtin_amdgcn_readfirstlane((a * g.U + iu) * g.V);
+            const float* const row0 = proj + base;
+            const float* const row1 = row0 + g.V;
             // Every geometry term of a voxel is a combination, with coefficients constant along the
             // column, of four per-voxel products: gb*Iv*dv, gb*Iu, gb*Iv and g*interp.  The column
             // sums those four and combines them once (the same terms as :214-292, summed in
@@ -359,8 +367,11 @@ __global__ __launch_bounds__(64 * BP_WAVES) void bp_backward_kernel(
                     const int iv = (int)vi;
                     const float fv = __builtin_amdgcn_fractf(vi), mv = 1.0f - fv;
                     const int k00 = base + iv;
-                    const float p00 = proj[k00], p01 = proj[k00 + 1];
-                    const float p10 = proj[k00 + g.V], p11 = proj[k00 + g.V + 1];
+                    const unsigned bo = 4u * (unsigned)iv;  // byte offset in the row (< 2^31)
+                    const float* const q0 = reinterpret_cast<const float*>(reinterpret_cast<const char*>(row0) + bo);
+                    const float* const q1 = reinterpret_cast<const float*>(reinterpret_cast<const char*>(row1) + bo);
+                    const float p00 = q0[0], p01 = q0[1];
+                    const float p10 = q1[0], p11 = q1[1];
                     const float interp = fmaf(fu * fv, p11, fmaf(mu * fv, p01, fmaf(fu * mv, p10, mu * mv * p00)));
                     const float Iu = fmaf(fv, p11 - p01, mv * (p10 - p00));
                     const float Iv = fmaf(fu, p11 - p10, mu * (p01 - p00));

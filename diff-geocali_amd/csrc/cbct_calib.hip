@@ -446,7 +446,7 @@ static int run_calibration(const cbct_calib_problem* pr, int nproblems, int nste
         total_tiles += s.nblocks;
         total_params += 8 * (int)q.A;
         max_cap = std::max(max_cap, s.cap);
-        big = big || q.D * q.H * q.W > INT32_MAX;
+        big = big || slab_wide_offsets(q.D, q.H, q.W);
     }
     CalibDev* ddev = (CalibDev*)(ws + L.descs);
     unsigned* counter = (unsigned*)(ws + L.counter);

@@ -8,6 +8,11 @@ namespace cbct {
 constexpr int SL_TILE = 16;
 constexpr int SL_THREADS = SL_TILE * SL_TILE;
 
+// The slab kernels' narrow variants take voxel offsets in 32 bits and volume byte offsets in 32
+// unsigned bits, so they serve volumes below 2^30 voxels (4 GiB); larger ones take the variant
+// with 64-bit offsets.
+inline bool slab_wide_offsets(int64_t D, int64_t H, int64_t W) { return D * H * W >= (int64_t(1) << 30); }
+
 struct SlabArgs {
     const float* vol;
     const float* grad;

@@ -45,7 +45,7 @@ __device__ __forceinline__ void pixel_dir(const ViewFrame& f, float u, float v, 
 // GEO (MODE 2 only): with false, the geometry gradients are not wanted -- the adjoint then needs
 // no volume taps at all (the scatter depends only on the sample positions and the upstream
 // weights): no staged volume box, no interpolation, no geometry sums, one LDS box per workgroup.
-// BIG: the volume has more than 2^31 - 1 voxels, so voxel offsets are formed in 64 bits (the
+// BIG: the volume has 2^30 or more voxels, so voxel offsets are formed in 64 bits (the
 // 32-bit forms of the other variants are cheaper in the staging and flush loops).
 template <int MODE, bool DIV, bool ZOFF = false, int CAPC = 0, bool GEO = true, bool BIG = false>
 __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
@@ -513,7 +513,7 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 const int y = min(max(lo[1] + (r - __mul24(z, B1)), 0), b.H - 1);
                 const int zz = min(max(lo[2] + z, 0), b.D - 1);
                 // zz*H + y < D*H < 2^31 (host check, D and H below 2^24); the whole offset is
-                // 64-bit when the volume exceeds 2^31 voxels (BIG)
+                // 64-bit when the volume has 2^30 voxels or more (BIG)
                 return (VoxelIndex)(__umul24((unsigned)zz, (unsigned)b.H) + (unsigned)y) * b.W + x;
             };
             // A box inside the volume needs no clamp, and a lane's elements i, i + 256, ... can be
@@ -566,9 +566,12 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                     const int i = i0 + lane;
                     if (TAPS) {
                         const VoxelIndex vi = walk ? (VoxelIndex)woff : box_voxel(i);
+                        // (without BIG the volume is below 4 GiB, so the byte offset is a 32-bit lane
+                        // offset from the kernel argument's address)
+                        const void* src = BIG ? (const void*)(P.vol + vi)
+                                              : (const void*)(reinterpret_cast<const char*>(P.vol) + 4u * (unsigned)vi);
                         if (i0 + 64 <= nbox)
-                            __builtin_amdgcn_global_load_lds(
-                                (const void*)(P.vol + vi), (__attribute__((address_space(3))) void*)(vbox + i0), 4, 0, 0);
+                            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(vbox + i0), 4, 0, 0);
                         else if (i < nbox)
                             vbox[i] = P.vol[vi];
                     }
@@ -597,7 +600,10 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                         const long long iv = sh >= 0 ? (q << min(sh, 62)) : (q >> min(-sh, 62));
                         if (iv != 0) atomicAdd(P.gacc + vi, (unsigned long long)iv);
                     } else {
-                        atomicAdd(P.gvol + vi, (float)ival * inv_scale);
+                        // (a 32-bit byte offset from the argument's address, as in the staging)
+                        float* dst = BIG ? P.gvol + vi
+                                         : reinterpret_cast<float*>(reinterpret_cast<char*>(P.gvol) + 4u * (unsigned)vi);
+                        atomicAdd(dst, (float)ival * inv_scale);
                     }
                 };
                 if (walk) {
@@ -776,7 +782,7 @@ int launch_fp_slab(int mode, bool div, const SlabArgs& args, hipStream_t stream,
     const size_t shmem = (size_t)args.cap * sizeof(float) * (mode == 2 && geometry ? 2 : 1);
     const dim3 grid((unsigned)args.nblocks), block(SL_THREADS);
     const bool zoff = args.b.offx == 0.f && args.b.offy == 0.f && args.b.offz == 0.f;
-    const bool big = (int64_t)args.b.D * args.b.H * args.b.W > INT32_MAX;
+    const bool big = slab_wide_offsets(args.b.D, args.b.H, args.b.W);
     if (big) {
         // 64-bit voxel offsets: one generic variant per mode
         if (mode == 0 && div) fp_slab_kernel<0, true, false, 0, true, true><<<grid, block, shmem, stream>>>(args);

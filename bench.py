@@ -76,14 +76,14 @@ DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 # WRITE_SIZE x 1.0 (KiB x 1024).  The counts depend only on the workload, the kernel and its tuning
 # (SQ_WAIT_INST_LDS and SQ_WAVE_CYCLES vary by about 1% between runs).
 MEASURED = {
-    ("geometry-gradients-256c-720v", "adjoint"): dict(valu=113_357_391_648, lds_conflict=38_899_405_936,
-                                                      lds_active=73_355_711_832, lds_wait=49_859_088_744,
-                                                      wave_qcycles=453_988_065_090, grbm=3_026_922_843,
-                                                      fetch_kib=100_220_168, write_kib=139_240_895),
-    ("geometry-gradients-256c-720v", "forward"): dict(valu=53_543_571_102, lds_conflict=9_670_306_160,
-                                                      lds_active=26_231_704_402, lds_wait=11_725_480_890,
-                                                      wave_qcycles=259_789_671_254, grbm=1_248_358_994,
-                                                      fetch_kib=90_301_252, write_kib=1_474_642),
+    ("geometry-gradients-256c-720v", "adjoint"): dict(valu=112_585_473_556, lds_conflict=38_899_405_936,
+                                                      lds_active=73_355_711_832, lds_wait=52_360_760_960,
+                                                      wave_qcycles=446_755_690_302, grbm=2_972_387_914,
+                                                      fetch_kib=99_793_888, write_kib=139_267_070),
+    ("geometry-gradients-256c-720v", "forward"): dict(valu=53_153_846_202, lds_conflict=9_670_306_160,
+                                                      lds_active=26_231_687_469, lds_wait=11_774_708_273,
+                                                      wave_qcycles=258_244_077_777, grbm=1_227_390_340,
+                                                      fetch_kib=90_421_904, write_kib=1_472_667),
     # one fused Adam march of the 64-problem batched calibration (all problems, one launch)
     (CALIB_WORKLOAD, "march"): dict(valu=1_255_542_653_659, lds_conflict=179_891_963_696, grbm=29_455_398_767,
                                     fetch_kib=3_137_745_506, write_kib=86_052_683),

@@ -85,18 +85,18 @@ MEASURED = {
                                                       wave_qcycles=258_244_077_777, grbm=1_227_390_340,
                                                       fetch_kib=90_421_904, write_kib=1_472_667),
     # one fused Adam march of the 64-problem batched calibration (all problems, one launch)
-    (CALIB_WORKLOAD, "march"): dict(valu=1_255_542_653_659, lds_conflict=179_891_963_696, grbm=29_455_398_767,
-                                    fetch_kib=3_137_745_506, write_kib=86_052_683),
+    (CALIB_WORKLOAD, "march"): dict(valu=1_248_901_289_303, lds_conflict=188_925_821_208, grbm=29_230_132_670,
+                                    fetch_kib=3_117_609_529, write_kib=86_090_254),
     # forward at 360 views, 512x384 (FDK round trip; HBM counters only); backprojection backward /
     # forward at 256^3 x 360 views
-    ("fdk-roundtrip-256c-360v", "forward"): dict(fetch_kib=47_404_480, write_kib=296_951),
-    ("backprojection-256c-360v", "bp_backward"): dict(valu=8_566_132_836, lds_conflict=896_532_168,
-                                                      lds_active=3_087_791_426, lds_wait=818_363_786,
-                                                      wave_qcycles=28_923_119_949, grbm=268_885_770,
-                                                      fetch_kib=1_683_878, write_kib=6_993_152),
-    ("backprojection-256c-360v", "bp_forward"): dict(valu=3_143_892_992, lds_conflict=0, lds_active=0, lds_wait=0,
-                                                     wave_qcycles=25_756_333_071, grbm=110_697_725,
-                                                     fetch_kib=8_417_371, write_kib=173_073),
+    ("fdk-roundtrip-256c-360v", "forward"): dict(fetch_kib=47_461_840, write_kib=296_819),
+    ("backprojection-256c-360v", "bp_backward"): dict(valu=8_252_738_194, lds_conflict=896_532_168,
+                                                      lds_active=3_079_634_000, lds_wait=817_399_235,
+                                                      wave_qcycles=27_536_107_527, grbm=258_441_513,
+                                                      fetch_kib=1_714_566, write_kib=7_138_081),
+    ("backprojection-256c-360v", "bp_forward"): dict(valu=2_672_033_792, lds_conflict=0, lds_active=0, lds_wait=0,
+                                                     wave_qcycles=25_513_154_256, grbm=106_890_065,
+                                                     fetch_kib=5_650_494, write_kib=168_070),
 }
 CUS, XCDS = 256, 8
 

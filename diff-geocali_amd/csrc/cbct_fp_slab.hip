@@ -553,6 +553,14 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 wy = cy ? wy - B1 : wy;
                 woff += wk0 + (cx ? wkx : (WalkIndex)0) + (cy ? wky : (WalkIndex)0);
             };
+            // A gathering kernel's box inside the volume needs no clamp either: its element i is
+            // at byte offset 4 (z plane + y W + x) from the box's first voxel, formed by 24-bit
+            // multiply-adds (exact while 4 H W < 2^24, the box's cell coordinates are below 2^13
+            // and the byte offsets below 2^32, i.e. without BIG): 15 full-rate VALU instructions per
+            // element instead of 23 with the clamps, one of them a quarter-rate 64-bit multiply-add
+            // (forward 64.1 -> 61.9 ms at 256^3 x 720, bit-identical).
+            const bool inner = MODE != 2 && !BIG && staged && lo[0] >= 0 && lo[1] >= 0 && lo[2] >= 0 &&
+                               hi[0] < b.W && hi[1] < b.H && hi[2] < b.D && plane < (1 << 22);
             if (staged) {
                 if (!BCAST) __syncthreads();
                 // Box element i goes to vbox[i]; wave w stages elements c + 64 w + lane of each
@@ -561,22 +569,65 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 // 4 l), so every load of the box is in flight at once and holds no register; a
                 // partial last run goes through registers.  The gradient box is zeroed alongside.
                 const int wbase = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
-                if (walk) walk_start();
-                for (int i0 = wbase; i0 < nbox; i0 += SL_THREADS) {
-                    const int i = i0 + lane;
-                    if (TAPS) {
-                        const VoxelIndex vi = walk ? (VoxelIndex)woff : box_voxel(i);
-                        // (without BIG the volume is below 4 GiB, so the byte offset is a 32-bit lane
-                        // offset from the kernel argument's address)
-                        const void* src = BIG ? (const void*)(P.vol + vi)
-                                              : (const void*)(reinterpret_cast<const char*>(P.vol) + 4u * (unsigned)vi);
-                        if (i0 + 64 <= nbox)
-                            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(vbox + i0), 4, 0, 0);
-                        else if (i < nbox)
-                            vbox[i] = P.vol[vi];
+                if constexpr (MODE == 2) {
+                    if (walk) walk_start();
+                    for (int i0 = wbase; i0 < nbox; i0 += SL_THREADS) {
+                        const int i = i0 + lane;
+                        if (TAPS) {
+                            const VoxelIndex vi = walk ? (VoxelIndex)woff : box_voxel(i);
+                            // (without BIG the volume is below 4 GiB, so the byte offset is a 32-bit
+                            // lane offset from the kernel argument's address)
+                            const void* src = BIG ? (const void*)(P.vol + vi)
+                                                  : (const void*)(reinterpret_cast<const char*>(P.vol) + 4u * (unsigned)vi);
+                            if (i0 + 64 <= nbox)
+                                __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(vbox + i0), 4, 0, 0);
+                            else if (i < nbox)
+                                vbox[i] = P.vol[vi];
+                        }
+                        if (i < nbox) gboxu[i] = 0u;
+                        if (walk) walk_next();
                     }
-                    if (MODE == 2 && i < nbox) gboxu[i] = 0u;
-                    if (walk) walk_next();
+                } else {
+                    // The gathering kernels' staging, with the element's byte offset from the
+                    // volume's address given by off_of (32-bit without BIG, so the loads take the
+                    // kernel argument's address as their scalar base).  (The same loop for the
+                    // volume-gradient adjoint measured 0.7 ms slower than the one above; this form
+                    // gives the forward 0.9 ms over that one.)
+                    auto stage = [&](auto off_of) {
+                        for (int i0 = wbase; i0 < nbox; i0 += SL_THREADS) {
+                            const int i = i0 + lane;
+                            auto off = off_of(i);
+                            if (i0 + 64 <= nbox) {
+                                __builtin_amdgcn_global_load_lds(reinterpret_cast<const char*>(P.vol) + off,
+                                                                 (__attribute__((address_space(3))) void*)(vbox + i0), 4, 0, 0);
+                            } else if (i < nbox) {
+                                // (an opaque copy: one 64-bit address shared with this load would
+                                // cost the loads above their scalar base)
+                                asm volatile("" : "+v"(off));
+                                vbox[i] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.vol) + off);
+                            }
+                        }
+                    };
+                    if (inner) {
+                        const unsigned plane4 = 4u * (unsigned)plane, W4 = 4u * (unsigned)b.W;
+                        const unsigned first4 = 4u * (unsigned)((lo[2] * b.H + lo[1]) * b.W + lo[0]);
+                        stage([&](int i) {
+                            const int r = (int)fmaf((float)i, rB0, 0.5f * rB0);
+                            const int x = i - __mul24(r, B0);
+                            const int z = (int)fmaf((float)r, rB1, 0.5f * rB1);
+                            int y = r - __mul24(z, B1);
+                            // (kept as computed: folded into the products, y W4 becomes a 32-bit
+                            // multiply of z by B1 W4, a quarter-rate instruction)
+                            asm volatile("" : "+v"(y));
+                            return __umul24((unsigned)z, plane4) + __umul24((unsigned)y, W4) + 4u * (unsigned)x + first4;
+                        });
+                    } else {
+                        stage([&](int i) {
+                            const VoxelIndex vi = box_voxel(i);
+                            using ByteOffset = typename std::conditional<BIG, int64_t, unsigned>::type;
+                            return BIG ? (ByteOffset)(4 * (int64_t)vi) : (ByteOffset)(4u * (unsigned)vi);
+                        });
+                    }
                 }
                 // s_waitcnt vmcnt(0): the direct-to-LDS loads have landed before the barrier
                 if (TAPS) __builtin_amdgcn_s_waitcnt(0x0F70);

@@ -76,14 +76,14 @@ DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 # WRITE_SIZE x 1.0 (KiB x 1024).  The counts depend only on the workload, the kernel and its tuning
 # (SQ_WAIT_INST_LDS and SQ_WAVE_CYCLES vary by about 1% between runs).
 MEASURED = {
-    ("geometry-gradients-256c-720v", "adjoint"): dict(valu=112_585_473_556, lds_conflict=38_899_405_936,
-                                                      lds_active=73_355_711_832, lds_wait=52_360_760_960,
-                                                      wave_qcycles=446_755_690_302, grbm=2_972_387_914,
-                                                      fetch_kib=99_793_888, write_kib=139_267_070),
-    ("geometry-gradients-256c-720v", "forward"): dict(valu=53_153_846_202, lds_conflict=9_670_306_160,
-                                                      lds_active=26_231_687_469, lds_wait=11_774_708_273,
-                                                      wave_qcycles=258_244_077_777, grbm=1_227_390_340,
-                                                      fetch_kib=90_421_904, write_kib=1_472_667),
+    ("geometry-gradients-256c-720v", "adjoint"): dict(valu=106_903_345_865, lds_conflict=38_899_405_936,
+                                                      lds_active=73_355_711_832, lds_wait=54_254_832_989,
+                                                      wave_qcycles=440_974_947_036, grbm=2_940_969_741,
+                                                      fetch_kib=101_587_462, write_kib=137_481_839),
+    ("geometry-gradients-256c-720v", "forward"): dict(valu=50_606_928_139, lds_conflict=9_670_306_160,
+                                                      lds_active=26_231_582_037, lds_wait=13_019_082_346,
+                                                      wave_qcycles=247_525_715_972, grbm=1_186_346_950,
+                                                      fetch_kib=90_741_066, write_kib=1_470_222),
     # one fused Adam march of the 64-problem batched calibration (all problems, one launch)
     (CALIB_WORKLOAD, "march"): dict(valu=1_248_901_289_303, lds_conflict=188_925_821_208, grbm=29_230_132_670,
                                     fetch_kib=3_117_609_529, write_kib=86_090_254),

@@ -516,51 +516,27 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 // 64-bit when the volume has 2^30 voxels or more (BIG)
                 return (VoxelIndex)(__umul24((unsigned)zz, (unsigned)b.H) + (unsigned)y) * b.W + x;
             };
-            // A box inside the volume needs no clamp, and a lane's elements i, i + 256, ... can be
-            // walked: from element i to i + 256 the x coordinate moves by rem = 256 mod B0 (one
-            // wrap at most, carrying into the row) and the row by q = 256 / B0 or q + 1 (one wrap
-            // of y at most when q + 1 <= B1, carrying into z), so the voxel offset moves by one of
-            // four uniform constants.  By induction the walked (x, y, offset) equal the element's
-            // own decomposition for every element of the box.  Used by the volume-gradient adjoint
-            // (staging and flush: 164 -> 161 ms at 256^3 x 720), not by the gathering kernels,
-            // whose forward it slows by about 1 ms.
-            const bool walk = MODE == 2 && staged && lo[0] >= 0 && lo[1] >= 0 && lo[2] >= 0 && hi[0] < b.W && hi[1] < b.H &&
-                              hi[2] < b.D && SL_THREADS / B0 + 1 <= B1;
-            // (unsigned: a lane's walk runs a few elements past the box's last one, whose offsets
-            // are never used and may exceed the 32-bit range of the offsets in use)
-            using WalkIndex = typename std::make_unsigned<VoxelIndex>::type;
-            const int wq = SL_THREADS / B0, wrem = SL_THREADS - wq * B0;
-            const WalkIndex plane_v = (WalkIndex)b.H * (WalkIndex)b.W;
-            const WalkIndex wk0 = (WalkIndex)wrem + (WalkIndex)wq * (WalkIndex)b.W, wkx = (WalkIndex)(b.W - B0);
-            const WalkIndex wky = plane_v - (WalkIndex)B1 * (WalkIndex)b.W;
-            int wx = 0, wy = 0;
-            WalkIndex woff = 0;
-            auto walk_start = [&]() {
-                const int i = threadIdx.x;
-                const int r = (int)fmaf((float)i, rB0, 0.5f * rB0);
-                const int z = (int)fmaf((float)r, rB1, 0.5f * rB1);
-                wx = i - __mul24(r, B0);
-                wy = r - __mul24(z, B1);
-                woff = ((WalkIndex)(lo[2] + z) * (WalkIndex)b.H + (WalkIndex)(lo[1] + wy)) * (WalkIndex)b.W +
-                       (WalkIndex)(lo[0] + wx);
-            };
-            auto walk_next = [&]() {
-                wx += wrem;
-                const bool cx = wx >= B0;
-                wx = cx ? wx - B0 : wx;
-                wy += cx ? wq + 1 : wq;
-                const bool cy = wy >= B1;
-                wy = cy ? wy - B1 : wy;
-                woff += wk0 + (cx ? wkx : (WalkIndex)0) + (cy ? wky : (WalkIndex)0);
-            };
-            // A gathering kernel's box inside the volume needs no clamp either: its element i is
-            // at byte offset 4 (z plane + y W + x) from the box's first voxel, formed by 24-bit
-            // multiply-adds (exact while 4 H W < 2^24, the box's cell coordinates are below 2^13
-            // and the byte offsets below 2^32, i.e. without BIG): 15 full-rate VALU instructions per
-            // element instead of 23 with the clamps, one of them a quarter-rate 64-bit multiply-add
-            // (forward 64.1 -> 61.9 ms at 256^3 x 720, bit-identical).
-            const bool inner = MODE != 2 && !BIG && staged && lo[0] >= 0 && lo[1] >= 0 && lo[2] >= 0 &&
+            // A box inside the volume needs no clamp: its element i is at offset z plane + y W + x
+            // from the box's first voxel, formed by 24-bit multiply-adds (exact while 4 H W < 2^24,
+            // the box's cell coordinates are below 2^13 and the byte offsets below 2^32, i.e.
+            // without BIG): 15 full-rate VALU instructions per element instead of 23 with the
+            // clamps, one of them a quarter-rate 64-bit multiply-add.  At 256^3 x 720: forward
+            // 64.1 -> 61.9 ms (bit-identical); volume-gradient adjoint 157.4 -> 155.3 ms against an
+            // incremental walk of each lane's elements i, i + 256, ... (about 24 instructions per
+            // element, its uniform steps selected per element), which it replaces.
+            const bool inner = !BIG && staged && lo[0] >= 0 && lo[1] >= 0 && lo[2] >= 0 &&
                                hi[0] < b.W && hi[1] < b.H && hi[2] < b.D && plane < (1 << 22);
+            const unsigned first = (unsigned)((lo[2] * b.H + lo[1]) * b.W + lo[0]);
+            auto inner_voxel = [&](int i) {      // element offset (the adjoint's staging and flush)
+                const int r = (int)fmaf((float)i, rB0, 0.5f * rB0);
+                const int x = i - __mul24(r, B0);
+                const int z = (int)fmaf((float)r, rB1, 0.5f * rB1);
+                int y = r - __mul24(z, B1);
+                // (kept as computed: folded into the products, y W becomes a 32-bit multiply of z
+                // by B1 W, a quarter-rate instruction)
+                asm volatile("" : "+v"(y));
+                return __umul24((unsigned)z, (unsigned)plane) + __umul24((unsigned)y, (unsigned)b.W) + (unsigned)x + first;
+            };
             if (staged) {
                 if (!BCAST) __syncthreads();
                 // Box element i goes to vbox[i]; wave w stages elements c + 64 w + lane of each
@@ -570,11 +546,10 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 // partial last run goes through registers.  The gradient box is zeroed alongside.
                 const int wbase = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
                 if constexpr (MODE == 2) {
-                    if (walk) walk_start();
                     for (int i0 = wbase; i0 < nbox; i0 += SL_THREADS) {
                         const int i = i0 + lane;
                         if (TAPS) {
-                            const VoxelIndex vi = walk ? (VoxelIndex)woff : box_voxel(i);
+                            const VoxelIndex vi = inner ? (VoxelIndex)inner_voxel(i) : box_voxel(i);
                             // (without BIG the volume is below 4 GiB, so the byte offset is a 32-bit
                             // lane offset from the kernel argument's address)
                             const void* src = BIG ? (const void*)(P.vol + vi)
@@ -585,7 +560,6 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                                 vbox[i] = P.vol[vi];
                         }
                         if (i < nbox) gboxu[i] = 0u;
-                        if (walk) walk_next();
                     }
                 } else {
                     // The gathering kernels' staging, with the element's byte offset from the
@@ -609,15 +583,14 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                         }
                     };
                     if (inner) {
-                        const unsigned plane4 = 4u * (unsigned)plane, W4 = 4u * (unsigned)b.W;
-                        const unsigned first4 = 4u * (unsigned)((lo[2] * b.H + lo[1]) * b.W + lo[0]);
+                        // (the byte offset formed directly: one instruction fewer than 4 x the
+                        // element offset)
+                        const unsigned plane4 = 4u * (unsigned)plane, W4 = 4u * (unsigned)b.W, first4 = 4u * first;
                         stage([&](int i) {
                             const int r = (int)fmaf((float)i, rB0, 0.5f * rB0);
                             const int x = i - __mul24(r, B0);
                             const int z = (int)fmaf((float)r, rB1, 0.5f * rB1);
                             int y = r - __mul24(z, B1);
-                            // (kept as computed: folded into the products, y W4 becomes a 32-bit
-                            // multiply of z by B1 W4, a quarter-rate instruction)
                             asm volatile("" : "+v"(y));
                             return __umul24((unsigned)z, plane4) + __umul24((unsigned)y, W4) + 4u * (unsigned)x + first4;
                         });
@@ -657,12 +630,10 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                         atomicAdd(dst, (float)ival * inv_scale);
                     }
                 };
-                if (walk) {
-                    walk_start();
+                if (inner) {
                     for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {
                         const int ival = (int)gboxu[i];
-                        if (ival != 0) flush(ival, (VoxelIndex)woff);
-                        walk_next();
+                        if (ival != 0) flush(ival, (VoxelIndex)inner_voxel(i));
                     }
                 } else {
                     for (int i = threadIdx.x; i < nbox; i += SL_THREADS) {

@@ -85,11 +85,11 @@ MEASURED = {
                                                       wave_qcycles=247_525_715_972, grbm=1_186_346_950,
                                                       fetch_kib=90_741_066, write_kib=1_470_222),
     # one fused Adam march of the 64-problem batched calibration (all problems, one launch)
-    (CALIB_WORKLOAD, "march"): dict(valu=1_248_901_289_303, lds_conflict=188_925_821_208, grbm=29_230_132_670,
-                                    fetch_kib=3_117_609_529, write_kib=86_090_254),
+    (CALIB_WORKLOAD, "march"): dict(valu=1_200_173_386_422, lds_conflict=186_517_640_901, grbm=28_545_396_381,
+                                    fetch_kib=3_122_219_376, write_kib=98_516_707),
     # forward at 360 views, 512x384 (FDK round trip; HBM counters only); backprojection backward /
     # forward at 256^3 x 360 views
-    ("fdk-roundtrip-256c-360v", "forward"): dict(fetch_kib=47_461_840, write_kib=296_819),
+    ("fdk-roundtrip-256c-360v", "forward"): dict(fetch_kib=47_329_416, write_kib=296_965),
     ("backprojection-256c-360v", "bp_backward"): dict(valu=8_252_738_194, lds_conflict=896_532_168,
                                                       lds_active=3_079_634_000, lds_wait=817_399_235,
                                                       wave_qcycles=27_536_107_527, grbm=258_441_513,

@@ -516,26 +516,25 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                 // 64-bit when the volume has 2^30 voxels or more (BIG)
                 return (VoxelIndex)(__umul24((unsigned)zz, (unsigned)b.H) + (unsigned)y) * b.W + x;
             };
-            // A box inside the volume needs no clamp: its element i is at offset z plane + y W + x
-            // from the box's first voxel, formed by 24-bit multiply-adds (exact while 4 H W < 2^24,
-            // the box's cell coordinates are below 2^13 and the byte offsets below 2^32, i.e.
-            // without BIG): 15 full-rate VALU instructions per element instead of 23 with the
-            // clamps, one of them a quarter-rate 64-bit multiply-add.  At 256^3 x 720: forward
-            // 64.1 -> 61.9 ms (bit-identical); volume-gradient adjoint 157.4 -> 155.3 ms against an
-            // incremental walk of each lane's elements i, i + 256, ... (about 24 instructions per
-            // element, its uniform steps selected per element), which it replaces.
+            // A box inside the volume needs no clamp: its element i = (z B1 + y) B0 + x, with
+            // r = z B1 + y its row, is at offset z plane + y W + x = i + r (W - B0) + z W (H - B1)
+            // from the box's first voxel -- two 24-bit multiply-adds with non-negative
+            // coefficients (exact while 4 H W < 2^24, the box's cell coordinates are below 2^13
+            // and the byte offsets below 2^32, i.e. without BIG).  10 full-rate VALU instructions
+            // per element instead of 23 with the clamps (one of them a quarter-rate 64-bit
+            // multiply-add).  At 256^3 x 720: forward 64.1 -> 61.9 ms with x and y formed
+            // explicitly (15 instructions), 60.2 ms in this form, bit-identical; volume-gradient
+            // adjoint 157.4 -> 155.3 ms against an incremental walk of each lane's elements i,
+            // i + 256, ... (about 24 instructions per element, its uniform steps selected per
+            // element), which it replaces.
             const bool inner = !BIG && staged && lo[0] >= 0 && lo[1] >= 0 && lo[2] >= 0 &&
                                hi[0] < b.W && hi[1] < b.H && hi[2] < b.D && plane < (1 << 22);
             const unsigned first = (unsigned)((lo[2] * b.H + lo[1]) * b.W + lo[0]);
+            const unsigned row_skip = (unsigned)(b.W - B0), plane_skip = (unsigned)b.W * (unsigned)(b.H - B1);
             auto inner_voxel = [&](int i) {      // element offset (the adjoint's staging and flush)
                 const int r = (int)fmaf((float)i, rB0, 0.5f * rB0);
-                const int x = i - __mul24(r, B0);
                 const int z = (int)fmaf((float)r, rB1, 0.5f * rB1);
-                int y = r - __mul24(z, B1);
-                // (kept as computed: folded into the products, y W becomes a 32-bit multiply of z
-                // by B1 W, a quarter-rate instruction)
-                asm volatile("" : "+v"(y));
-                return __umul24((unsigned)z, (unsigned)plane) + __umul24((unsigned)y, (unsigned)b.W) + (unsigned)x + first;
+                return __umul24((unsigned)z, plane_skip) + __umul24((unsigned)r, row_skip) + (unsigned)i + first;
             };
             if (staged) {
                 if (!BCAST) __syncthreads();
@@ -585,14 +584,11 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
                     if (inner) {
                         // (the byte offset formed directly: one instruction fewer than 4 x the
                         // element offset)
-                        const unsigned plane4 = 4u * (unsigned)plane, W4 = 4u * (unsigned)b.W, first4 = 4u * first;
+                        const unsigned row4 = 4u * row_skip, plane4 = 4u * plane_skip, first4 = 4u * first;
                         stage([&](int i) {
                             const int r = (int)fmaf((float)i, rB0, 0.5f * rB0);
-                            const int x = i - __mul24(r, B0);
                             const int z = (int)fmaf((float)r, rB1, 0.5f * rB1);
-                            int y = r - __mul24(z, B1);
-                            asm volatile("" : "+v"(y));
-                            return __umul24((unsigned)z, plane4) + __umul24((unsigned)y, W4) + 4u * (unsigned)x + first4;
+                            return __umul24((unsigned)z, plane4) + __umul24((unsigned)r, row4) + 4u * (unsigned)i + first4;
                         });
                     } else {
                         stage([&](int i) {

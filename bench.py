@@ -76,14 +76,14 @@ DEFAULT_WORKLOAD = "geometry-gradients-256c-720v"
 # WRITE_SIZE x 1.0 (KiB x 1024).  The counts depend only on the workload, the kernel and its tuning
 # (SQ_WAIT_INST_LDS and SQ_WAVE_CYCLES vary by about 1% between runs).
 MEASURED = {
-    ("geometry-gradients-256c-720v", "adjoint"): dict(valu=106_903_345_865, lds_conflict=38_899_405_936,
-                                                      lds_active=73_355_711_832, lds_wait=54_254_832_989,
-                                                      wave_qcycles=440_974_947_036, grbm=2_940_969_741,
-                                                      fetch_kib=101_587_462, write_kib=137_481_839),
-    ("geometry-gradients-256c-720v", "forward"): dict(valu=50_606_928_139, lds_conflict=9_670_306_160,
-                                                      lds_active=26_231_582_037, lds_wait=13_019_082_346,
-                                                      wave_qcycles=247_525_715_972, grbm=1_186_346_950,
-                                                      fetch_kib=90_741_066, write_kib=1_470_222),
+    ("geometry-gradients-256c-720v", "adjoint"): dict(valu=104_597_247_185, lds_conflict=38_899_405_936,
+                                                      lds_active=73_355_711_832, lds_wait=56_032_938_653,
+                                                      wave_qcycles=438_637_946_358, grbm=2_935_886_323,
+                                                      fetch_kib=101_631_588, write_kib=137_481_984),
+    ("geometry-gradients-256c-720v", "forward"): dict(valu=48_996_622_309, lds_conflict=9_670_306_160,
+                                                      lds_active=26_231_495_702, lds_wait=14_547_534_868,
+                                                      wave_qcycles=241_074_440_990, grbm=1_157_686_989,
+                                                      fetch_kib=91_054_548, write_kib=1_470_551),
     # one fused Adam march of the 64-problem batched calibration (all problems, one launch)
     (CALIB_WORKLOAD, "march"): dict(valu=1_200_173_386_422, lds_conflict=186_517_640_901, grbm=28_545_396_381,
                                     fetch_kib=3_122_219_376, write_kib=98_516_707),

@@ -529,7 +529,8 @@ __device__ __forceinline__ void slab_tile(const SlabArgs& P, int tile) {
             // element), which it replaces.
             const bool inner = !BIG && staged && lo[0] >= 0 && lo[1] >= 0 && lo[2] >= 0 &&
                                hi[0] < b.W && hi[1] < b.H && hi[2] < b.D && plane < (1 << 22);
-            const unsigned first = (unsigned)((lo[2] * b.H + lo[1]) * b.W + lo[0]);
+            // (unsigned arithmetic: the value is used only when inner, where it is below 2^30)
+            const unsigned first = ((unsigned)lo[2] * (unsigned)b.H + (unsigned)lo[1]) * (unsigned)b.W + (unsigned)lo[0];
             const unsigned row_skip = (unsigned)(b.W - B0), plane_skip = (unsigned)b.W * (unsigned)(b.H - B1);
             auto inner_voxel = [&](int i) {      // element offset (the adjoint's staging and flush)
                 const int r = (int)fmaf((float)i, rB0, 0.5f * rB0);

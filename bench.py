@@ -85,11 +85,11 @@ MEASURED = {
                                                       wave_qcycles=241_074_440_990, grbm=1_157_686_989,
                                                       fetch_kib=91_054_548, write_kib=1_470_551),
     # one fused Adam march of the 64-problem batched calibration (all problems, one launch)
-    (CALIB_WORKLOAD, "march"): dict(valu=1_200_173_386_422, lds_conflict=186_517_640_901, grbm=28_545_396_381,
-                                    fetch_kib=3_122_219_376, write_kib=98_516_707),
+    (CALIB_WORKLOAD, "march"): dict(valu=1_166_453_661_623, lds_conflict=179_186_528_075, grbm=28_067_277_858,
+                                    fetch_kib=3_145_644_934, write_kib=98_684_285),
     # forward at 360 views, 512x384 (FDK round trip; HBM counters only); backprojection backward /
     # forward at 256^3 x 360 views
-    ("fdk-roundtrip-256c-360v", "forward"): dict(fetch_kib=47_329_416, write_kib=296_965),
+    ("fdk-roundtrip-256c-360v", "forward"): dict(fetch_kib=47_285_859, write_kib=296_987),
     ("backprojection-256c-360v", "bp_backward"): dict(valu=8_252_738_194, lds_conflict=896_532_168,
                                                       lds_active=3_079_634_000, lds_wait=817_399_235,
                                                       wave_qcycles=27_536_107_527, grbm=258_441_513,
